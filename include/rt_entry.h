@@ -1,0 +1,175 @@
+/*
+ * rt_entry for MI355X (gfx950): C ABI of the query path
+ *   PCA query projection -> entry-leaf selection -> greedy beam search.
+ *
+ * Conventions
+ *  - Every call returns RTE_OK (0) or a negative RTE_E* code; the message of
+ *    the last failure on a context is rte_last_error(ctx).  Nothing calls
+ *    exit() (the reference prints and exits, helper.h:4-34).
+ *  - One context owns one HIP stream and the device buffers it allocated.
+ *    Calls on one context are not thread-safe; separate contexts are.
+ *  - `where` says whether a pointer argument is host (RTE_HOST) or device
+ *    (RTE_DEVICE) memory.  Host inputs are copied into context-owned device
+ *    memory.  Device inputs to the rte_set_* calls are borrowed, not copied:
+ *    the caller keeps them alive and unchanged until the context is destroyed
+ *    or the slot is set again.  Host outputs are written before the call
+ *    returns; device outputs are written in stream order on the context's
+ *    stream (rte_synchronize() waits for them).
+ *  - Ids are int32, distances are squared L2 in FP32.
+ */
+#ifndef RT_ENTRY_H
+#define RT_ENTRY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTE_OK 0
+#define RTE_EINVAL (-1)   /* bad argument or shape */
+#define RTE_ESTATE (-2)   /* a required rte_set_* call is missing */
+#define RTE_EHIP (-3)     /* HIP runtime error */
+#define RTE_ENOMEM (-4)   /* device allocation failed */
+#define RTE_EUNSUP (-5)   /* shape not compiled into this build */
+#define RTE_EBUILD (-6)   /* host-side build (kd-tree split) failed */
+
+#define RTE_HOST 0
+#define RTE_DEVICE 1
+
+#define RTE_MODE_REF 0   /* reference semantics: dist-only strict compares */
+#define RTE_MODE_TUNED 1 /* (dist, id) compares, auto-sized visited filter */
+
+typedef struct rte_ctx rte_ctx;
+
+/* Search parameters (graph.cu:32-64 Graph ctor, flags.cpp defaults). */
+typedef struct rte_params {
+  int k;            /* results per query, 1 <= k <= beam (topk) */
+  int beam;         /* n_candidates; rounded up to a power of two, 32..512 */
+  int width;        /* search_width: graph rows expanded per iteration */
+  int max_iter;     /* iteration cap (flags.cpp max_iter, default 100) */
+  int mode;         /* RTE_MODE_REF or RTE_MODE_TUNED */
+  int algo;         /* 1: leaf entry table; 0 or 2: per-lane LCG entries */
+  int bloom_words;  /* 0: default (REF 512, TUNED auto_tune_bloom sizing) */
+  int bloom_hashes; /* 0: default (REF 3, TUNED auto_tune_bloom sizing) */
+} rte_params;
+
+/* ---- context ---------------------------------------------------------- */
+/* Replaces Graph::Graph + cublasCreate (graph.cu:32-64). */
+int rte_create(rte_ctx** ctx, int device);
+/* Replaces Graph::CleanUp / OptiXRT::CleanUp (graph.cu:339-342,
+ * raytracing.cpp:263-281). */
+int rte_destroy(rte_ctx* ctx);
+const char* rte_last_error(const rte_ctx* ctx);
+/* Use the caller's HIP stream (hipStream_t) instead of the context's own;
+ * NULL restores the context stream. */
+int rte_set_stream(rte_ctx* ctx, void* stream);
+void* rte_get_stream(rte_ctx* ctx);
+int rte_synchronize(rte_ctx* ctx);
+
+/* ---- data ------------------------------------------------------------- */
+/* Search-space vectors, np x d row-major with row stride `stride` floats
+ * (PCA-rotated or raw).  Replaces Graph::Input / Projection's upload of
+ * d_points_ / d_pca_points (graph.cu:73-112, 180-186). */
+int rte_set_base(rte_ctx* ctx, const float* base, int64_t np, int d, int64_t stride,
+                 int where);
+/* Proximity graph rows, np x degree int32, padded with np.  Rows are
+ * re-laid out to a power-of-two stride (graph.cu:134 offset_shift) when
+ * degree is not one.  Replaces Graph::RB_Graph (graph.cu:114-139). */
+int rte_set_graph(rte_ctx* ctx, const int32_t* rows, int64_t np, int degree, int where);
+/* PCA rotation R (d x r_cols row-major; column j = eigenvector j) and mean
+ * mu (d); the projection keeps the first d_out columns.  Replaces
+ * Graph::Projection's device setup (graph.cu:188-228: copy of R[:, :DIM],
+ * mu.R GEMM and replicateVector). */
+int rte_set_pca(rte_ctx* ctx, const float* R, const float* mu, int d, int r_cols,
+                int d_out, int where);
+/* PQ = Q R[:, :d_out] - mu R[:, :d_out] as an FP32 MFMA GEMM.  Replaces
+ * matrixMultiply(handle_, d_queries_, d_rotation, d_pca_queries, nq, DIM,
+ * dim_, 1, -1) (graph.cu:262-263, matrix.cu:39-69). */
+int rte_project(rte_ctx* ctx, const float* Q, int64_t nq, float* PQ, int where);
+
+/* ---- entry selection --------------------------------------------------- */
+/* Leaf boxes (n_leaf x {minx,miny,minz,maxx,maxy,maxz}, already expanded)
+ * and the leaf point table (n_leaf x cap ascending ids, padded with np).
+ * Builds the traversal hierarchy over the boxes.  Replaces
+ * RT_Entry::InitRT -> OptiXRT::Setup/BuildAccel (entry.cu:79-98,
+ * raytracing.cpp:184-261) and the aabb_entries upload (entry.cu:60-69). */
+int rte_set_leaves(rte_ctx* ctx, const float* aabbs, const int32_t* pids, int n_leaf,
+                   int cap, int where);
+/* Host kd-tree over 3-D points (np x 3): leaves of at most max_node points
+ * in left-first DFS order; writes unexpanded boxes (n_leaf x 6, capacity
+ * max_leaves) and the ascending pid table (max_leaves x max_node).
+ * Replaces Kdtree + computeAabbPid (kdtree.cpp:4-195) in
+ * RT_Entry::BlockUp (entry.cu:8-77). */
+int rte_build_kdtree(const float* pts3, int64_t np, int max_node, int max_leaves,
+                     float* aabbs_out, int32_t* pids_out, int* n_leaf_out);
+/* expandAabb (aabb.cu:27-38), host memory. */
+void rte_expand_aabbs(float* aabbs, int n_leaf, float ratio);
+/* hits[q] = smallest leaf whose box holds (PQ[q*stride+0..2]), 0 if none.
+ * Replaces RT_Entry::Search -> OptiXRT::search (entry.cu:100-149,
+ * raytracing.cpp:283-334, devicePrograms.cu:29-96). */
+int rte_select_entry(rte_ctx* ctx, const float* PQ, int64_t nq, int stride,
+                     int32_t* leaf_out, int where);
+
+/* ---- search ------------------------------------------------------------ */
+/* Greedy beam search of nq queries (row stride q_stride floats) from their
+ * entry leaves.  ids_out/dists_out are nq x k.  Replaces
+ * Graph::GraphSearch -> GraphSearchKernel (graph.cu:295-337,
+ * graph_search.cuh:56-1964); distances are an addition.  stats_out (may be
+ * NULL) receives nq x 3 int32: distance evaluations, graph rows read,
+ * iterations. */
+int rte_search(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stride,
+               const int32_t* leaf, const rte_params* p, int32_t* ids_out,
+               float* dists_out, int32_t* stats_out, int where);
+/* Batched uneven search: per-query k and beam (beam rounded up to a power
+ * of two and to at least k), one persistent launch with a work queue.
+ * Outputs have row stride k_stride >= max k.  No reference counterpart; it
+ * generalises GraphSearchKernel's launch (graph.cu:320-323) to mixed k and
+ * beam.  p->k and p->beam are ignored. */
+int rte_search_batched(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stride,
+                       const int32_t* leaf, const int32_t* k_per_q,
+                       const int32_t* beam_per_q, const rte_params* p, int k_stride,
+                       int32_t* ids_out, float* dists_out, int where);
+
+/* Whole query path on device buffers (graph.cu:254-293 Graph::Search):
+ * project, select entries (or take the predictor's leaves), search. */
+int rte_query(rte_ctx* ctx, const float* Q, int64_t nq, const rte_params* p,
+              float* PQ_scratch, int32_t* leaf_scratch, int32_t* ids_out,
+              float* dists_out);
+
+/* ---- learned entry predictor (no reference counterpart) ---------------- */
+/* MLP d_in -> h1 -> h2 -> n_leaf, ReLU, bf16 weights, FP32 accumulation. */
+int rte_mlp_create(rte_ctx* ctx, int d_in, int h1, int h2, int n_out, uint64_t seed);
+/* One SGD-momentum step on a device batch (X: n x d_in f32, y: n int32
+ * labels); writes the mean cross-entropy loss to *loss (host). */
+int rte_mlp_train_step(rte_ctx* ctx, const float* X, const int32_t* y, int n, float lr,
+                       float momentum, float* loss);
+/* Predicted leaf and softmax confidence per query (device buffers).
+ * leaf_out[q] falls back to fallback[q] (the kd-tree leaf) when the
+ * softmax maximum is below min_conf. */
+int rte_mlp_predict(rte_ctx* ctx, const float* X, int64_t n, const int32_t* fallback,
+                    float min_conf, int32_t* leaf_out, float* conf_out);
+/* Weights in/out as FP32 host arrays (W1 d_in x h1, b1, W2 h1 x h2, b2,
+ * W3 h2 x n_out, b3), for checkpoints and the gradient check. */
+int rte_mlp_get(rte_ctx* ctx, float* W1, float* b1, float* W2, float* b2, float* W3,
+                float* b3);
+int rte_mlp_set(rte_ctx* ctx, const float* W1, const float* b1, const float* W2,
+                const float* b2, const float* W3, const float* b3);
+/* Gradients of the mean cross-entropy of one device batch with respect to
+ * the FP32 master weights (host outputs, same shapes as rte_mlp_get). */
+int rte_mlp_grad(rte_ctx* ctx, const float* X, const int32_t* y, int n, float* gW1,
+                 float* gb1, float* gW2, float* gb2, float* gW3, float* gb3,
+                 float* loss);
+
+/* ---- misc -------------------------------------------------------------- */
+/* auto_tune_bloom.h:67-89 hash_parameter, used to size the TUNED filter. */
+void rte_hash_parameter(int nc, int* hash_len, int* bit, int* hash);
+/* Build configuration: 1 if (beam, batch, dims-per-lane) has a compiled
+ * search kernel. */
+int rte_search_supported(int beam, int batch, int d);
+const char* rte_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

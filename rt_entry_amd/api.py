@@ -1,0 +1,243 @@
+"""Python host API over the C ABI.
+
+`Context` mirrors include/rt_entry.h one call for one call.  Arguments may be
+numpy arrays (host memory: RTE_HOST) or torch CUDA tensors (device memory:
+RTE_DEVICE); device tensors passed to the set_* calls are borrowed, so the
+context keeps a reference to them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (MODE_REF, MODE_TUNED, RTE_DEVICE, RTE_HOST, Params, RteError, check, lib)
+
+__all__ = ["Context", "Params", "make_params", "MODE_REF", "MODE_TUNED", "RteError",
+           "build_kdtree", "expand_aabbs", "hash_parameter", "leaf_capacity"]
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch") and hasattr(x, "is_cuda")
+
+
+def _ptr(x, dtype):
+    """(pointer, where, keepalive) of a numpy array or torch tensor."""
+    if x is None:
+        return None, RTE_HOST, None
+    if _is_torch(x):
+        import torch
+        want = {np.float32: torch.float32, np.int32: torch.int32}[dtype]
+        if x.dtype != want:
+            raise TypeError(f"expected {want}, got {x.dtype}")
+        if not x.is_contiguous():
+            raise ValueError("device tensors must be contiguous")
+        if not x.is_cuda:
+            a = x.numpy()
+            return a.ctypes.data_as(C.c_void_p), RTE_HOST, a
+        return C.c_void_p(x.data_ptr()), RTE_DEVICE, x
+    a = np.ascontiguousarray(x, dtype=dtype)
+    return a.ctypes.data_as(C.c_void_p), RTE_HOST, a
+
+
+def make_params(k=10, beam=64, width=1, max_iter=100, mode=MODE_REF, algo=1, bloom_words=0,
+                bloom_hashes=0) -> Params:
+    return Params(int(k), int(beam), int(width), int(max_iter), int(mode), int(algo),
+                  int(bloom_words), int(bloom_hashes))
+
+
+def hash_parameter(nc: int):
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    lib().rte_hash_parameter(int(nc), C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value
+
+
+def leaf_capacity(np_: int, point_ratio: float) -> int:
+    """entry.cu:22: aabb_size = np * point_ratio in float32, truncated."""
+    return int(np.float32(np_) * np.float32(point_ratio))
+
+
+def build_kdtree(pts3, max_node: int, max_leaves: int | None = None):
+    """Host kd-tree (librte): returns (unexpanded boxes [n_leaf,6], pids
+    [n_leaf,max_node] ascending, padded with np)."""
+    pts = np.ascontiguousarray(pts3, dtype=np.float32)
+    n = pts.shape[0]
+    if max_leaves is None:
+        max_leaves = max(16, 4 * (n // max(1, max_node)) + 16)
+    aabbs = np.zeros((max_leaves, 6), np.float32)
+    pids = np.zeros((max_leaves, max_node), np.int32)
+    nl = C.c_int()
+    rc = lib().rte_build_kdtree(pts.ctypes.data_as(C.c_void_p), n, int(max_node), max_leaves,
+                                aabbs.ctypes.data_as(C.c_void_p), pids.ctypes.data_as(C.c_void_p),
+                                C.byref(nl))
+    if rc != 0:
+        raise RteError(rc, "rte_build_kdtree failed")
+    return aabbs[: nl.value].copy(), pids[: nl.value].copy()
+
+
+def expand_aabbs(aabbs, ratio: float):
+    a = np.array(aabbs, dtype=np.float32, copy=True, order="C")
+    lib().rte_expand_aabbs(a.ctypes.data_as(C.c_void_p), a.shape[0], float(ratio))
+    return a
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        rc = lib().rte_create(C.byref(self._h), int(device))
+        if rc != 0:
+            raise RteError(rc, f"rte_create(device={device}) failed")
+        self._keep = {}
+        self.d = 0
+        self.d_out = 0
+        self.np = 0
+
+    def close(self):
+        if self._h:
+            lib().rte_destroy(self._h)
+            self._h = C.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        return check(self._h, rc)
+
+    @property
+    def stream(self) -> int:
+        return int(lib().rte_get_stream(self._h) or 0)
+
+    def set_stream(self, stream_ptr: int | None):
+        self._chk(lib().rte_set_stream(self._h, C.c_void_p(stream_ptr or 0)))
+
+    def synchronize(self):
+        self._chk(lib().rte_synchronize(self._h))
+
+    # ---- data
+    def set_base(self, base, d: int | None = None):
+        p, w, keep = _ptr(base, np.float32)
+        n, stride = base.shape
+        d = stride if d is None else d
+        self._chk(lib().rte_set_base(self._h, p, n, d, stride, w))
+        self._keep["base"] = keep if w == RTE_DEVICE else None
+        self.d, self.np = d, n
+
+    def set_graph(self, rows):
+        p, w, keep = _ptr(rows, np.int32)
+        n, deg = rows.shape
+        self._chk(lib().rte_set_graph(self._h, p, n, deg, w))
+        self._keep["graph"] = keep if w == RTE_DEVICE else None
+
+    def set_pca(self, R, mu, d_out: int):
+        pr, w, k1 = _ptr(R, np.float32)
+        pm, w2, k2 = _ptr(mu, np.float32)
+        if w != w2:
+            raise ValueError("R and mu must live in the same memory")
+        d, rc = R.shape
+        self._chk(lib().rte_set_pca(self._h, pr, pm, d, rc, int(d_out), w))
+        self.d_out = int(d_out)
+
+    def project(self, Q, out=None):
+        pq, w, keep = _ptr(Q, np.float32)
+        nq = Q.shape[0]
+        if out is None:
+            if w == RTE_DEVICE:
+                import torch
+                out = torch.empty((nq, self.d_out), dtype=torch.float32, device=Q.device)
+            else:
+                out = np.empty((nq, self.d_out), np.float32)
+        po, wo, _ = _ptr(out, np.float32)
+        if wo != w:
+            raise ValueError("input and output must live in the same memory")
+        self._chk(lib().rte_project(self._h, pq, nq, po, w))
+        return out
+
+    def set_leaves(self, aabbs_expanded, pids):
+        pa, w, k1 = _ptr(aabbs_expanded, np.float32)
+        pp, w2, k2 = _ptr(pids, np.int32)
+        if w != w2:
+            raise ValueError("boxes and pids must live in the same memory")
+        n_leaf, cap = pids.shape
+        self._chk(lib().rte_set_leaves(self._h, pa, pp, n_leaf, cap, w))
+        self._keep["pids"] = k2 if w == RTE_DEVICE else None
+
+    def select_entry(self, PQ, out=None):
+        pq, w, keep = _ptr(PQ, np.float32)
+        nq, stride = PQ.shape
+        if out is None:
+            if w == RTE_DEVICE:
+                import torch
+                out = torch.empty(nq, dtype=torch.int32, device=PQ.device)
+            else:
+                out = np.empty(nq, np.int32)
+        po, wo, _ = _ptr(out, np.int32)
+        self._chk(lib().rte_select_entry(self._h, pq, nq, stride, po, w))
+        return out
+
+    # ---- search
+    def search(self, PQ, leaf, params: Params, with_dists=True, with_stats=False, out=None):
+        pq, w, keep = _ptr(PQ, np.float32)
+        nq, qs = PQ.shape
+        k = params.k
+        if out is None:
+            if w == RTE_DEVICE:
+                import torch
+                dev = PQ.device
+                ids = torch.empty((nq, k), dtype=torch.int32, device=dev)
+                dists = torch.empty((nq, k), dtype=torch.float32, device=dev) if with_dists else None
+                stats = torch.empty((nq, 3), dtype=torch.int32, device=dev) if with_stats else None
+            else:
+                ids = np.empty((nq, k), np.int32)
+                dists = np.empty((nq, k), np.float32) if with_dists else None
+                stats = np.empty((nq, 3), np.int32) if with_stats else None
+        else:
+            ids, dists, stats = out
+        pl, _, _ = _ptr(leaf, np.int32) if leaf is not None else (None, w, None)
+        pi, _, _ = _ptr(ids, np.int32)
+        pd, _, _ = _ptr(dists, np.float32) if dists is not None else (None, w, None)
+        ps, _, _ = _ptr(stats, np.int32) if stats is not None else (None, w, None)
+        self._chk(lib().rte_search(self._h, pq, nq, qs, pl, C.byref(params), pi, pd, ps, w))
+        res = [ids]
+        if with_dists:
+            res.append(dists)
+        if with_stats:
+            res.append(stats)
+        return tuple(res) if len(res) > 1 else ids
+
+    def search_batched(self, PQ, leaf, k_per_q, beam_per_q, params: Params, k_stride=None,
+                       out=None):
+        pq, w, keep = _ptr(PQ, np.float32)
+        nq, qs = PQ.shape
+        if k_stride is None:
+            k_stride = int(k_per_q.max())
+        if out is None:
+            if w == RTE_DEVICE:
+                import torch
+                ids = torch.full((nq, k_stride), -1, dtype=torch.int32, device=PQ.device)
+                dists = torch.full((nq, k_stride), float("nan"), dtype=torch.float32,
+                                   device=PQ.device)
+            else:
+                ids = np.full((nq, k_stride), -1, np.int32)
+                dists = np.full((nq, k_stride), np.nan, np.float32)
+        else:
+            ids, dists = out
+        pl, _, _ = _ptr(leaf, np.int32) if leaf is not None else (None, w, None)
+        pk, wk, k1 = _ptr(k_per_q, np.int32)
+        pb, wb, k2 = _ptr(beam_per_q, np.int32)
+        if wk != w or wb != w:
+            raise ValueError("k_per_q / beam_per_q must live with the queries")
+        pi, _, _ = _ptr(ids, np.int32)
+        pd, _, _ = _ptr(dists, np.float32)
+        self._chk(lib().rte_search_batched(self._h, pq, nq, qs, pl, pk, pb, C.byref(params),
+                                           int(k_stride), pi, pd, w))
+        return ids, dists

@@ -1,0 +1,570 @@
+// C ABI of the MI355X query path (include/rt_entry.h): context, device
+// buffers, argument checks and launches.  Host memory arguments are staged
+// through context-owned device buffers; device arguments are used in place.
+#include "rt_entry.h"
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "ctx.h"
+#include "entry.h"
+#include "kdtree.h"
+#include "project.h"
+#include "search.h"
+
+namespace {
+
+void hash_parameter(int nc, int* hash_len, int* bit, int* hash);
+
+}  // namespace
+
+int rte_fail(rte_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+namespace {
+
+#define fail rte_fail
+
+int hip_fail(rte_ctx* c, hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory)
+    return fail(c, RTE_ENOMEM, "%s: %s", what, hipGetErrorString(e));
+  return fail(c, RTE_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, expr, what)                      \
+  do {                                               \
+    hipError_t e_ = (expr);                          \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, what); \
+  } while (0)
+
+bool is_pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
+int pow2_ceil(int x) {
+  int p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+int ilog2(int x) {
+  int s = 0;
+  while ((1 << s) < x) ++s;
+  return s;
+}
+
+void hash_parameter(int nc, int* hash_len, int* bit, int* hash) {
+  // auto_tune_bloom.h:67-89: n = 12.897 nc; the smallest m = 32 * 2^i
+  // (i = 1..11) whose false-positive rate with k = round(ln2 m/n) is < 3%.
+  const double n = (double)nc * 12.897;
+  const int maxn = 11;
+  for (int i = 1; i <= maxn; ++i) {
+    const double m = 32.0 * (double)(1 << i);
+    const double r = m / n;
+    const double k = std::round(std::log(2.0) * r);
+    const double p = std::pow(1.0 - std::exp(-k / r), k);
+    if (p < 0.03) {
+      *hash_len = 1 << i;
+      *bit = i;
+      *hash = (int)std::round(k);
+      return;
+    }
+    if (i == maxn) *hash_len = 1 << i;  // the reference's unbraced if
+    *bit = i;
+    *hash = (int)std::round(k);
+  }
+}
+
+// Resolve the filter configuration for one beam.
+void bloom_for(const rte_params* p, int beam, int* words, int* hashes) {
+  if (p->bloom_words > 0) {
+    *words = p->bloom_words;
+    *hashes = p->bloom_hashes > 0 ? p->bloom_hashes : 3;
+    return;
+  }
+  if (p->mode == RTE_MODE_TUNED) {
+    int hl, bit, hh;
+    hash_parameter(beam, &hl, &bit, &hh);
+    *words = hl;
+    *hashes = std::min(hh, 10);
+    return;
+  }
+  *words = 512;  // graph_search.cuh:4-5
+  *hashes = p->bloom_hashes > 0 ? p->bloom_hashes : 3;
+}
+
+int check_params(rte_ctx* c, const rte_params* p, bool per_query) {
+  if (!p) return fail(c, RTE_EINVAL, "params is NULL");
+  if (!per_query) {
+    if (p->beam < 1 || p->beam > 512)
+      return fail(c, RTE_EINVAL, "beam %d outside 1..512", p->beam);
+    if (p->k < 1 || p->k > pow2_ceil(std::max(p->beam, 32)))
+      return fail(c, RTE_EINVAL, "k %d must be in 1..beam", p->k);
+  }
+  if (p->width < 1 || p->width > rte::kMaxWidth)
+    return fail(c, RTE_EINVAL, "width %d outside 1..%d", p->width, rte::kMaxWidth);
+  if (p->max_iter < 0) return fail(c, RTE_EINVAL, "max_iter < 0");
+  if (p->mode != RTE_MODE_REF && p->mode != RTE_MODE_TUNED)
+    return fail(c, RTE_EINVAL, "unknown mode %d", p->mode);
+  if (p->algo < 0 || p->algo > 2) return fail(c, RTE_EINVAL, "unknown algo %d", p->algo);
+  if (p->bloom_words != 0 &&
+      (!is_pow2(p->bloom_words) || p->bloom_words < 2 || p->bloom_words > 2048))
+    return fail(c, RTE_EINVAL, "bloom_words must be a power of two in 2..2048");
+  if (p->bloom_hashes < 0 || p->bloom_hashes > 10)
+    return fail(c, RTE_EINVAL, "bloom_hashes must be in 0..10");
+  return RTE_OK;
+}
+
+// device pointer for an input that may live on the host
+template <typename T>
+int stage_in(rte_ctx* c, DevBuf& buf, const T* src, size_t count, int where, const T** dev) {
+  if (where == RTE_DEVICE) {
+    *dev = src;
+    return RTE_OK;
+  }
+  HIPCHK(c, buf.reserve(count * sizeof(T)), "hipMalloc(staging)");
+  HIPCHK(c, hipMemcpyAsync(buf.p, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream),
+         "hipMemcpyAsync H2D");
+  *dev = buf.as<T>();
+  return RTE_OK;
+}
+
+template <typename T>
+int stage_out(rte_ctx* c, DevBuf& buf, T* dst, size_t count, int where, T** dev) {
+  if (where == RTE_DEVICE || dst == nullptr) {
+    *dev = dst;
+    return RTE_OK;
+  }
+  HIPCHK(c, buf.reserve(count * sizeof(T)), "hipMalloc(staging)");
+  *dev = buf.as<T>();
+  return RTE_OK;
+}
+
+template <typename T>
+int finish_out(rte_ctx* c, T* host, const T* dev, size_t count, int where) {
+  if (where == RTE_DEVICE || host == nullptr) return RTE_OK;
+  HIPCHK(c, hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost, c->stream),
+         "hipMemcpyAsync D2H");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return RTE_OK;
+}
+
+int set_device(rte_ctx* c) {
+  HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+  return RTE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rte_create(rte_ctx** out, int device) {
+  if (!out) return RTE_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return RTE_EHIP;
+  if (device < 0 || device >= n) return RTE_EINVAL;
+  rte_ctx* c = new rte_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return RTE_EHIP;
+  }
+  c->stream = c->own;
+  *out = c;
+  return RTE_OK;
+}
+
+int rte_destroy(rte_ctx* c) {
+  if (!c) return RTE_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+  return RTE_OK;
+}
+
+const char* rte_last_error(const rte_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int rte_set_stream(rte_ctx* c, void* stream) {
+  if (!c) return RTE_EINVAL;
+  c->stream = stream ? (hipStream_t)stream : c->own;
+  return RTE_OK;
+}
+
+void* rte_get_stream(rte_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int rte_synchronize(rte_ctx* c) {
+  if (!c) return RTE_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return RTE_OK;
+}
+
+int rte_set_base(rte_ctx* c, const float* base, int64_t np, int d, int64_t stride, int where) {
+  if (!c) return RTE_EINVAL;
+  if (!base || np <= 0 || np >= 0x7fffffff || d <= 0 || stride < d)
+    return fail(c, RTE_EINVAL, "bad base shape np=%lld d=%d stride=%lld", (long long)np, d,
+                (long long)stride);
+  if (d > 960) return fail(c, RTE_EUNSUP, "d=%d above 960", d);
+  if (int r = set_device(c)) return r;
+  if (where == RTE_HOST) {
+    const size_t bytes = (size_t)np * stride * sizeof(float);
+    HIPCHK(c, c->base_own.reserve(bytes), "hipMalloc(base)");
+    HIPCHK(c, hipMemcpyAsync(c->base_own.p, base, bytes, hipMemcpyHostToDevice, c->stream),
+           "copy base");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "sync base");
+    c->base = c->base_own.as<float>();
+  } else {
+    c->base_own.release();
+    c->base = base;
+  }
+  c->np = np;
+  c->d = d;
+  c->stride = stride;
+  return RTE_OK;
+}
+
+__global__ void repack_graph_kernel(const int32_t* __restrict__ src, int64_t np, int degree,
+                                    int stride, int32_t* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np * stride) return;
+  const int64_t row = i / stride;
+  const int col = (int)(i - row * stride);
+  dst[i] = col < degree ? src[row * degree + col] : (int32_t)np;
+}
+
+int rte_set_graph(rte_ctx* c, const int32_t* rows, int64_t np, int degree, int where) {
+  if (!c) return RTE_EINVAL;
+  if (!rows || np <= 0 || np >= 0x7fffffff || degree < 1 || degree > 64)
+    return fail(c, RTE_EINVAL, "bad graph shape np=%lld degree=%d (1..64)", (long long)np, degree);
+  if (int r = set_device(c)) return r;
+  const int D = pow2_ceil(degree);
+  const int shift = ilog2(D);
+  if (where == RTE_HOST) {
+    for (int64_t i = 0; i < np * degree; ++i)
+      if (rows[i] < 0 || rows[i] > np)
+        return fail(c, RTE_EINVAL, "graph id %d at %lld outside [0,np]", rows[i], (long long)i);
+    std::vector<int32_t> padded;
+    const int32_t* src = rows;
+    if (D != degree) {
+      padded.assign((size_t)np * D, (int32_t)np);
+      for (int64_t r = 0; r < np; ++r)
+        memcpy(&padded[(size_t)r * D], rows + r * degree, sizeof(int32_t) * degree);
+      src = padded.data();
+    }
+    const size_t bytes = (size_t)np * D * sizeof(int32_t);
+    HIPCHK(c, c->graph_own.reserve(bytes), "hipMalloc(graph)");
+    HIPCHK(c, hipMemcpyAsync(c->graph_own.p, src, bytes, hipMemcpyHostToDevice, c->stream),
+           "copy graph");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "sync graph");
+    c->graph = c->graph_own.as<int32_t>();
+  } else if (D != degree) {
+    const size_t bytes = (size_t)np * D * sizeof(int32_t);
+    HIPCHK(c, c->graph_own.reserve(bytes), "hipMalloc(graph)");
+    const int64_t tot = np * D;
+    hipLaunchKernelGGL(repack_graph_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       c->stream, rows, np, degree, D, c->graph_own.as<int32_t>());
+    HIPCHK(c, hipGetLastError(), "repack graph");
+    c->graph = c->graph_own.as<int32_t>();
+  } else {
+    c->graph_own.release();
+    c->graph = rows;
+  }
+  c->graph_np = np;
+  c->degree = degree;
+  c->shift = shift;
+  return RTE_OK;
+}
+
+int rte_set_pca(rte_ctx* c, const float* R, const float* mu, int d, int r_cols, int d_out,
+                int where) {
+  if (!c) return RTE_EINVAL;
+  if (!R || !mu || d <= 0 || d > 4096 || r_cols < d_out || d_out <= 0)
+    return fail(c, RTE_EINVAL, "bad pca shape d=%d r_cols=%d d_out=%d", d, r_cols, d_out);
+  if (int r = set_device(c)) return r;
+  HIPCHK(c, c->rot.reserve((size_t)d * d_out * sizeof(float)), "hipMalloc(rot)");
+  HIPCHK(c, c->bias.reserve((size_t)d_out * sizeof(float)), "hipMalloc(bias)");
+  const hipMemcpyKind kind = where == RTE_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  // keep the first d_out columns (graph.cu:196 CopyHostToDevice(..., DIM))
+  HIPCHK(c, hipMemcpy2DAsync(c->rot.p, d_out * sizeof(float), R, r_cols * sizeof(float),
+                             d_out * sizeof(float), d, kind, c->stream),
+         "copy rotation");
+  DevBuf mu_dev;
+  HIPCHK(c, mu_dev.reserve(d * sizeof(float)), "hipMalloc(mu)");
+  HIPCHK(c, hipMemcpyAsync(mu_dev.p, mu, d * sizeof(float), kind, c->stream), "copy mu");
+  HIPCHK(c, rte::launch_mu_r(mu_dev.as<float>(), c->rot.as<float>(), d, d_out, c->bias.as<float>(),
+                             c->stream),
+         "mu.R");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "sync pca");
+  c->pca_d = d;
+  c->pca_out = d_out;
+  return RTE_OK;
+}
+
+int rte_project(rte_ctx* c, const float* Q, int64_t nq, float* PQ, int where) {
+  if (!c) return RTE_EINVAL;
+  if (!c->pca_d) return fail(c, RTE_ESTATE, "rte_set_pca not called");
+  if (!Q || !PQ || nq < 0) return fail(c, RTE_EINVAL, "bad projection arguments");
+  if (nq == 0) return RTE_OK;
+  if (int r = set_device(c)) return r;
+  const float* dq;
+  float* dpq;
+  if (int r = stage_in(c, c->s_in, Q, (size_t)nq * c->pca_d, where, &dq)) return r;
+  if (int r = stage_out(c, c->s_out, PQ, (size_t)nq * c->pca_out, where, &dpq)) return r;
+  HIPCHK(c, rte::launch_project(dq, nq, c->pca_d, c->rot.as<float>(), c->pca_out,
+                                c->bias.as<float>(), dpq, c->stream),
+         "project");
+  return finish_out(c, PQ, dpq, (size_t)nq * c->pca_out, where);
+}
+
+int rte_build_kdtree(const float* pts3, int64_t np, int max_node, int max_leaves,
+                     float* aabbs_out, int32_t* pids_out, int* n_leaf_out) {
+  if (!pts3 || !aabbs_out || !pids_out || !n_leaf_out || np <= 0 || max_node <= 0 ||
+      max_leaves <= 0)
+    return RTE_EINVAL;
+  const int r = rte::build_kdtree(pts3, np, max_node, max_leaves, aabbs_out, pids_out, n_leaf_out);
+  if (r == -2) return RTE_EINVAL;
+  if (r == -3) return RTE_EBUILD;
+  return r == 0 ? RTE_OK : RTE_EINVAL;
+}
+
+void rte_expand_aabbs(float* aabbs, int n_leaf, float ratio) {
+  if (aabbs && n_leaf > 0) rte::expand_aabbs(aabbs, n_leaf, ratio);
+}
+
+int rte_set_leaves(rte_ctx* c, const float* aabbs, const int32_t* pids, int n_leaf, int cap,
+                   int where) {
+  if (!c) return RTE_EINVAL;
+  if (!aabbs || !pids || n_leaf <= 0 || cap <= 0)
+    return fail(c, RTE_EINVAL, "bad leaves n_leaf=%d cap=%d", n_leaf, cap);
+  if (int r = set_device(c)) return r;
+  std::vector<float> hb;
+  const float* boxes = aabbs;
+  if (where == RTE_DEVICE) {
+    hb.resize((size_t)n_leaf * 6);
+    HIPCHK(c, hipMemcpy(hb.data(), aabbs, hb.size() * sizeof(float), hipMemcpyDeviceToHost),
+           "read boxes");
+    boxes = hb.data();
+  }
+  std::vector<rte::BoxNode> nodes;
+  rte::build_hierarchy(boxes, n_leaf, nodes);
+  HIPCHK(c, c->nodes.reserve(nodes.size() * sizeof(rte::BoxNode)), "hipMalloc(nodes)");
+  HIPCHK(c, hipMemcpyAsync(c->nodes.p, nodes.data(), nodes.size() * sizeof(rte::BoxNode),
+                           hipMemcpyHostToDevice, c->stream),
+         "copy nodes");
+  if (where == RTE_HOST) {
+    const size_t bytes = (size_t)n_leaf * cap * sizeof(int32_t);
+    HIPCHK(c, c->pids_own.reserve(bytes), "hipMalloc(pids)");
+    HIPCHK(c, hipMemcpyAsync(c->pids_own.p, pids, bytes, hipMemcpyHostToDevice, c->stream),
+           "copy pids");
+    c->pids = c->pids_own.as<int32_t>();
+  } else {
+    c->pids_own.release();
+    c->pids = pids;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream), "sync leaves");
+  c->n_nodes = (int)nodes.size();
+  c->n_leaf = n_leaf;
+  c->cap = cap;
+  return RTE_OK;
+}
+
+int rte_select_entry(rte_ctx* c, const float* PQ, int64_t nq, int stride, int32_t* leaf_out,
+                     int where) {
+  if (!c) return RTE_EINVAL;
+  if (!c->n_nodes) return fail(c, RTE_ESTATE, "rte_set_leaves not called");
+  if (!PQ || !leaf_out || nq < 0 || stride < 3) return fail(c, RTE_EINVAL, "bad entry args");
+  if (nq == 0) return RTE_OK;
+  if (int r = set_device(c)) return r;
+  const float* dpq;
+  int32_t* dl;
+  if (int r = stage_in(c, c->s_in, PQ, (size_t)nq * stride, where, &dpq)) return r;
+  if (int r = stage_out(c, c->s_out, leaf_out, (size_t)nq, where, &dl)) return r;
+  HIPCHK(c, rte::launch_select_entry(c->nodes.as<rte::BoxNode>(), c->n_nodes, dpq, nq, stride, dl,
+                                     c->stream),
+         "select_entry");
+  return finish_out(c, leaf_out, dl, (size_t)nq, where);
+}
+
+static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride,
+                         const int32_t* leaf, const int32_t* k_per_q, const int32_t* beam_per_q,
+                         const rte_params* p, int k_stride, int32_t* ids_out, float* dists_out,
+                         int32_t* stats_out, int where) {
+  const bool uneven = beam_per_q != nullptr;
+  if (int r = check_params(c, p, uneven)) return r;
+  if (!c->base) return fail(c, RTE_ESTATE, "rte_set_base not called");
+  if (!c->graph) return fail(c, RTE_ESTATE, "rte_set_graph not called");
+  if (c->graph_np != c->np)
+    return fail(c, RTE_EINVAL, "graph has %lld rows, base %lld", (long long)c->graph_np,
+                (long long)c->np);
+  if (p->algo == 1 && !c->pids) return fail(c, RTE_ESTATE, "rte_set_leaves not called");
+  if (!PQ || !ids_out || nq < 0 || q_stride < c->d)
+    return fail(c, RTE_EINVAL, "bad search args (q_stride %lld < d %d?)", (long long)q_stride, c->d);
+  if (p->algo == 1 && !leaf) return fail(c, RTE_EINVAL, "leaf array required for algo 1");
+  if (nq == 0) return RTE_OK;
+  if (int r = set_device(c)) return r;
+
+  rte::SearchArgs a{};
+  a.base = c->base;
+  a.np = c->np;
+  a.stride = c->stride;
+  a.d = c->d;
+  a.graph = c->graph;
+  a.shift = c->shift;
+  a.q_stride = q_stride;
+  a.nq = nq;
+  a.pids = c->pids;
+  a.cap = c->cap;
+  a.n_leaf = c->n_leaf;
+  a.width = p->width;
+  a.max_iter = p->max_iter;
+  a.mode = p->mode;
+  a.algo = p->algo == 1 ? 1 : 0;
+  const int m = p->width << c->shift;
+  int beam_max;
+  std::vector<int32_t> hk, hb;
+  if (!uneven) {
+    const int beam = pow2_ceil(std::max(p->beam, 1));
+    if (beam < 32) return fail(c, RTE_EUNSUP, "beam below 32 is not compiled");
+    a.beam = beam;
+    a.k = p->k;
+    a.k_stride = p->k;
+    beam_max = beam;
+    int w, h;
+    bloom_for(p, beam, &w, &h);
+    a.bloom_words = w;
+    a.bloom_hashes = h;
+    a.bloom_words_max = w;
+  } else {
+    // per-query shapes: validate on the host copy
+    hk.resize(nq);
+    hb.resize(nq);
+    const hipMemcpyKind kind = where == RTE_HOST ? hipMemcpyHostToHost : hipMemcpyDeviceToHost;
+    HIPCHK(c, hipMemcpy(hk.data(), k_per_q, nq * sizeof(int32_t), kind), "read k_per_q");
+    HIPCHK(c, hipMemcpy(hb.data(), beam_per_q, nq * sizeof(int32_t), kind), "read beam_per_q");
+    beam_max = 32;
+    int kmax = 1;
+    for (int64_t i = 0; i < nq; ++i) {
+      if (hk[i] < 1 || hb[i] < 1 || hk[i] > k_stride)
+        return fail(c, RTE_EINVAL, "query %lld: k=%d beam=%d k_stride=%d", (long long)i, hk[i],
+                    hb[i], k_stride);
+      hb[i] = pow2_ceil(std::max(std::max(hb[i], hk[i]), 32));
+      if (hb[i] > 512) return fail(c, RTE_EINVAL, "query %lld: beam above 512", (long long)i);
+      beam_max = std::max(beam_max, hb[i]);
+      kmax = std::max(kmax, hk[i]);
+    }
+    a.k_stride = k_stride;
+    a.beam = beam_max;
+    a.k = kmax;
+    if (p->mode == RTE_MODE_TUNED && p->bloom_words == 0) {
+      a.bloom_auto = 1;
+      int wmax = 0;
+      for (int lg = 5; lg <= 9; ++lg) {
+        int w, h;
+        bloom_for(p, 1 << lg, &w, &h);
+        a.auto_words[lg] = w;
+        a.auto_hashes[lg] = h;
+        wmax = std::max(wmax, w);
+      }
+      a.bloom_words_max = wmax;
+    } else {
+      int w, h;
+      bloom_for(p, beam_max, &w, &h);
+      a.bloom_words = w;
+      a.bloom_hashes = h;
+      a.bloom_words_max = w;
+    }
+  }
+  if (!rte::search_supported(beam_max, m, c->d))
+    return fail(c, RTE_EUNSUP, "no kernel for beam %d batch %d d %d", beam_max, m, c->d);
+
+  const float* dq;
+  const int32_t* dleaf = nullptr;
+  int32_t* dids;
+  float* dd = nullptr;
+  int32_t* dst = nullptr;
+  if (int r = stage_in(c, c->s_in, PQ, (size_t)nq * q_stride, where, &dq)) return r;
+  if (p->algo == 1)
+    if (int r = stage_in(c, c->s_aux, leaf, (size_t)nq, where, &dleaf)) return r;
+  if (int r = stage_out(c, c->s_out, ids_out, (size_t)nq * a.k_stride, where, &dids)) return r;
+  if (dists_out)
+    if (int r = stage_out(c, c->s_aux2, dists_out, (size_t)nq * a.k_stride, where, &dd)) return r;
+  if (stats_out)
+    if (int r = stage_out(c, c->s_aux3, stats_out, (size_t)nq * 3, where, &dst)) return r;
+  DevBuf kq, bq;
+  if (uneven) {
+    HIPCHK(c, kq.reserve(nq * sizeof(int32_t)), "hipMalloc(k_per_q)");
+    HIPCHK(c, bq.reserve(nq * sizeof(int32_t)), "hipMalloc(beam_per_q)");
+    HIPCHK(c, hipMemcpyAsync(kq.p, hk.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice, c->stream),
+           "copy k_per_q");
+    HIPCHK(c, hipMemcpyAsync(bq.p, hb.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice, c->stream),
+           "copy beam_per_q");
+    a.k_per_q = kq.as<int32_t>();
+    a.beam_per_q = bq.as<int32_t>();
+  }
+  a.queries = dq;
+  a.hits = dleaf;
+  a.ids_out = dids;
+  a.dists_out = dd;
+  a.stats_out = dst;
+  const int64_t blocks = (nq + 3) / 4;
+  const int grid = (int)std::min<int64_t>(blocks, 1 << 20);
+  HIPCHK(c, rte::launch_search(a, beam_max, grid, c->stream), "search launch");
+  if (uneven) HIPCHK(c, hipStreamSynchronize(c->stream), "sync batched");
+  if (int r = finish_out(c, ids_out, dids, (size_t)nq * a.k_stride, where)) return r;
+  if (dists_out)
+    if (int r = finish_out(c, dists_out, dd, (size_t)nq * a.k_stride, where)) return r;
+  if (stats_out)
+    if (int r = finish_out(c, stats_out, dst, (size_t)nq * 3, where)) return r;
+  return RTE_OK;
+}
+
+int rte_search(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride, const int32_t* leaf,
+               const rte_params* p, int32_t* ids_out, float* dists_out, int32_t* stats_out,
+               int where) {
+  if (!c) return RTE_EINVAL;
+  return search_common(c, PQ, nq, q_stride, leaf, nullptr, nullptr, p, p ? p->k : 0, ids_out,
+                       dists_out, stats_out, where);
+}
+
+int rte_search_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride,
+                       const int32_t* leaf, const int32_t* k_per_q, const int32_t* beam_per_q,
+                       const rte_params* p, int k_stride, int32_t* ids_out, float* dists_out,
+                       int where) {
+  if (!c) return RTE_EINVAL;
+  if (!k_per_q || !beam_per_q) return fail(c, RTE_EINVAL, "k_per_q and beam_per_q required");
+  return search_common(c, PQ, nq, q_stride, leaf, k_per_q, beam_per_q, p, k_stride, ids_out,
+                       dists_out, nullptr, where);
+}
+
+int rte_query(rte_ctx* c, const float* Q, int64_t nq, const rte_params* p, float* PQ_scratch,
+              int32_t* leaf_scratch, int32_t* ids_out, float* dists_out) {
+  if (!c) return RTE_EINVAL;
+  if (int r = rte_project(c, Q, nq, PQ_scratch, RTE_DEVICE)) return r;
+  if (p && p->algo == 1)
+    if (int r = rte_select_entry(c, PQ_scratch, nq, c->pca_out, leaf_scratch, RTE_DEVICE)) return r;
+  return rte_search(c, PQ_scratch, nq, c->pca_out, leaf_scratch, p, ids_out, dists_out, nullptr,
+                    RTE_DEVICE);
+}
+
+void rte_hash_parameter(int nc, int* hash_len, int* bit, int* hash) {
+  hash_parameter(nc, hash_len, bit, hash);
+}
+
+int rte_search_supported(int beam, int batch, int d) {
+  return rte::search_supported(pow2_ceil(std::max(beam, 32)), batch, d);
+}
+
+const char* rte_version(void) { return "rt_entry_amd 0.1 gfx950"; }
+
+}  // extern "C"

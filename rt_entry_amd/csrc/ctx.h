@@ -1,0 +1,82 @@
+// Context object behind the C ABI (shared by the API translation units).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "rt_entry.h"
+
+namespace rte {
+struct Mlp;
+}
+
+// Owned device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // grows only; contents are not preserved
+  hipError_t reserve(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, n ? n : 1);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    bytes = n;
+    return hipSuccess;
+  }
+  template <typename T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+struct rte_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // search space
+  const float* base = nullptr;
+  int64_t np = 0;
+  int d = 0;
+  int64_t stride = 0;
+  DevBuf base_own;
+
+  // graph
+  const int32_t* graph = nullptr;
+  int64_t graph_np = 0;
+  int degree = 0;
+  int shift = 0;
+  DevBuf graph_own;
+
+  // projection
+  DevBuf rot;   // d_in x d_out compact
+  DevBuf bias;  // d_out: mu R
+  int pca_d = 0, pca_out = 0;
+
+  // leaves
+  DevBuf nodes;
+  int n_nodes = 0;
+  const int32_t* pids = nullptr;
+  DevBuf pids_own;
+  int n_leaf = 0, cap = 0;
+
+  // staging for host-pointer calls
+  DevBuf s_in, s_out, s_aux, s_aux2, s_aux3;
+
+  rte::Mlp* mlp = nullptr;
+  ~rte_ctx();
+};
+
+// record a failure message on the context and return code
+int rte_fail(rte_ctx* c, int code, const char* fmt, ...);

@@ -1,0 +1,90 @@
+// Entry-leaf selection: stackless traversal of a box hierarchy over the leaf
+// AABBs, one lane per query.
+//
+// Replaces the OptiX ray cast (raytracing.cpp:283-334, devicePrograms.cu:29-96):
+// the reference casts a zero-length ray from (pq0,pq1,pq2) into a BVH over
+// the expanded leaf boxes and keeps the first box reported.  CDNA4 has no
+// ray-tracing units; the leaves are already spatially ordered (left-first
+// DFS of the kd-tree, kdtree.cpp:193-194), so a binary hierarchy over leaf
+// index ranges, laid out in preorder with skip links, visits leaves in
+// ascending id order and the first containing leaf is the smallest such id.
+// A miss yields leaf 0 (entry.cu:88-89 fills hits with 0).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "entry.h"
+
+namespace rte {
+
+__global__ __launch_bounds__(256) void select_entry_kernel(const BoxNode* __restrict__ nodes,
+                                                           int n_nodes,
+                                                           const float* __restrict__ pq,
+                                                           int64_t nq, int stride,
+                                                           int32_t* __restrict__ hits) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const float x = pq[q * stride + 0];
+  const float y = pq[q * stride + 1];
+  const float z = pq[q * stride + 2];
+  int hit = 0;
+  int i = 0;
+  while (i < n_nodes) {
+    const BoxNode nd = nodes[i];
+    const bool in = x >= nd.lo[0] && x <= nd.hi[0] && y >= nd.lo[1] && y <= nd.hi[1] &&
+                    z >= nd.lo[2] && z <= nd.hi[2];
+    if (in) {
+      if (nd.leaf >= 0) {
+        hit = nd.leaf;
+        break;
+      }
+      ++i;  // first child follows in preorder
+    } else {
+      i = nd.skip;  // skip links always point forward, so the walk ends
+    }
+  }
+  hits[q] = hit;
+}
+
+hipError_t launch_select_entry(const BoxNode* nodes, int n_nodes, const float* pq, int64_t nq,
+                               int stride, int32_t* hits, hipStream_t st) {
+  if (nq <= 0) return hipSuccess;
+  const int64_t grid = (nq + 255) / 256;
+  hipLaunchKernelGGL(select_entry_kernel, dim3((unsigned)grid), dim3(256), 0, st, nodes, n_nodes,
+                     pq, nq, stride, hits);
+  return hipGetLastError();
+}
+
+// Preorder hierarchy over leaves [lo, hi): node box = union of leaf boxes.
+static int build_rec(const float* aabbs, int lo, int hi, std::vector<BoxNode>& out) {
+  const int me = (int)out.size();
+  out.push_back(BoxNode{});
+  if (hi - lo == 1) {
+    BoxNode& n = out[me];
+    for (int a = 0; a < 3; ++a) {
+      n.lo[a] = aabbs[(size_t)lo * 6 + a];
+      n.hi[a] = aabbs[(size_t)lo * 6 + 3 + a];
+    }
+    n.leaf = lo;
+  } else {
+    const int mid = lo + (hi - lo) / 2;
+    const int l = build_rec(aabbs, lo, mid, out);
+    const int r = build_rec(aabbs, mid, hi, out);
+    BoxNode& n = out[me];
+    for (int a = 0; a < 3; ++a) {
+      n.lo[a] = fminf(out[l].lo[a], out[r].lo[a]);
+      n.hi[a] = fmaxf(out[l].hi[a], out[r].hi[a]);
+    }
+    n.leaf = -1;
+  }
+  out[me].skip = (int)out.size();
+  return me;
+}
+
+void build_hierarchy(const float* aabbs, int n_leaf, std::vector<BoxNode>& out) {
+  out.clear();
+  if (n_leaf <= 0) return;
+  out.reserve((size_t)2 * n_leaf);
+  build_rec(aabbs, 0, n_leaf, out);
+}
+
+}  // namespace rte

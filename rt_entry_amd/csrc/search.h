@@ -1,0 +1,48 @@
+// Launch interface of the beam-search kernel (search.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rte {
+
+constexpr float kMaxD = 536870912.0f;  // (float)0x1fffffff, structure_on_device.cuh:6
+constexpr int kMaxWidth = 4;           // search_width rows per iteration
+
+struct SearchArgs {
+  const float* base;
+  int64_t np;
+  int64_t stride;
+  int d;
+  const int32_t* graph;
+  int shift;  // log2 of the graph row stride
+  const float* queries;
+  int64_t q_stride;
+  int64_t nq;
+  const int32_t* pids;  // n_leaf x cap
+  int cap;
+  int n_leaf;
+  const int32_t* hits;
+  int32_t* ids_out;
+  float* dists_out;
+  int32_t* stats_out;
+  int k;
+  int beam;
+  int k_stride;
+  int width;
+  int max_iter;
+  int mode;
+  int algo;
+  int bloom_words;
+  int bloom_hashes;
+  int bloom_words_max;  // LDS words reserved per wave for the filter
+  int bloom_auto;       // per-query sizing from auto_words/auto_hashes
+  int auto_words[16];   // indexed by log2(beam)
+  int auto_hashes[16];
+  const int32_t* k_per_q;     // uneven batches (or null)
+  const int32_t* beam_per_q;  // uneven batches (or null)
+};
+
+int search_supported(int beam_max, int m, int d);
+hipError_t launch_search(const SearchArgs& a, int beam_max, int grid, hipStream_t st);
+
+}  // namespace rte
