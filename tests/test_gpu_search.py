@@ -1,0 +1,196 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same FP32 inputs.  Ids and distances must match bit-for-bit."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import assert_same
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx_for(ctx, w):
+    ctx.set_base(w.base)
+    ctx.set_graph(w.graph_raw)
+    ctx.set_leaves(w.aabbs, w.pids)
+    return ctx
+
+
+def _run_both(ctx, w, queries, hits, **kw):
+    from rt_entry_amd import make_params
+    mode = kw.get("mode", 0)
+    beam = kw.get("beam", 64)
+    bw, bh = O.default_bloom(mode, 1 << (beam - 1).bit_length())
+    po = O.make_params(**{**kw, "bloom_words": kw.get("bloom_words", bw),
+                          "bloom_hashes": kw.get("bloom_hashes", bh)})
+    ids_o, d_o, st_o = O.search(w.base, w.graph, queries, w.pids, hits, po, with_stats=True)
+    pg = make_params(**kw)
+    ids_g, d_g, st_g = ctx.search(queries, hits, pg, with_dists=True, with_stats=True)
+    return (ids_o, d_o, st_o), (ids_g, d_g, st_g)
+
+
+@pytest.mark.parametrize("beam", [32, 64, 128, 256])
+def test_search_parity_beams(ctx, small_workload, beam):
+    w = small_workload
+    _ctx_for(ctx, w)
+    print(f"[parity] beam={beam} n={w.n} nq={w.queries.shape[0]}", flush=True)
+    (io, do, so), (ig, dg, sg) = _run_both(ctx, w, w.queries, w.hits, k=10, beam=beam)
+    assert_same(ig, io, "ids")
+    assert_same(dg.view(np.int32), do.view(np.int32), "dists")
+    assert_same(sg, so.astype(np.int32), "stats")
+
+
+@pytest.mark.parametrize("width", [1, 2, 4])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_search_parity_width_mode(ctx, int_workload, width, mode):
+    w = int_workload
+    _ctx_for(ctx, w)
+    print(f"[parity] width={width} mode={mode}", flush=True)
+    (io, do, so), (ig, dg, sg) = _run_both(ctx, w, w.queries, w.hits, k=10, beam=64,
+                                           width=width, mode=mode)
+    assert_same(ig, io, "ids")
+    assert_same(dg.view(np.int32), do.view(np.int32), "dists")
+
+
+def test_search_parity_k_equals_beam(ctx, int_workload):
+    w = int_workload
+    _ctx_for(ctx, w)
+    (io, do, _), (ig, dg, _) = _run_both(ctx, w, w.queries, w.hits, k=128, beam=128, max_iter=60)
+    assert_same(ig, io, "ids")
+    assert_same(dg.view(np.int32), do.view(np.int32), "dists")
+
+
+@pytest.mark.parametrize("algo", [0, 2])
+def test_search_parity_random_entry(ctx, small_workload, algo):
+    w = small_workload
+    _ctx_for(ctx, w)
+    (io, do, _), (ig, dg, _) = _run_both(ctx, w, w.queries, None, k=10, beam=64, algo=algo)
+    assert_same(ig, io, "ids")
+    assert_same(dg.view(np.int32), do.view(np.int32), "dists")
+
+
+def test_search_parity_degree48(ctx, deg48_workload):
+    w = deg48_workload
+    _ctx_for(ctx, w)
+    for beam, width in ((32, 1), (64, 2), (128, 1)):
+        print(f"[parity] deg48 beam={beam} width={width}", flush=True)
+        (io, do, _), (ig, dg, _) = _run_both(ctx, w, w.queries, w.hits, k=10, beam=beam,
+                                             width=width)
+        assert_same(ig, io, f"ids beam={beam} width={width}")
+        assert_same(dg.view(np.int32), do.view(np.int32), "dists")
+
+
+def test_search_parity_max_iter_and_bloom(ctx, small_workload):
+    w = small_workload
+    _ctx_for(ctx, w)
+    for kw in (dict(max_iter=0), dict(max_iter=1), dict(max_iter=7),
+               dict(bloom_words=64, bloom_hashes=2), dict(bloom_words=2048, bloom_hashes=5)):
+        print(f"[parity] {kw}", flush=True)
+        (io, do, _), (ig, dg, _) = _run_both(ctx, w, w.queries, w.hits, k=10, beam=64, **kw)
+        assert_same(ig, io, f"ids {kw}")
+        assert_same(dg.view(np.int32), do.view(np.int32), f"dists {kw}")
+
+
+def test_search_parity_truncated_dims(ctx, small_workload):
+    """Search on the first 64 PCA columns of a 128-wide base (row stride 128,
+    the reference's DIM < d layout) and on d = 100 (partial lane groups)."""
+    from rt_entry_amd import make_params
+    w = small_workload
+    for d in (64, 100):
+        ctx.set_base(w.base, d=d)
+        ctx.set_graph(w.graph_raw)
+        ctx.set_leaves(w.aabbs, w.pids)
+        po = O.make_params(k=10, beam=64)
+        q = np.ascontiguousarray(w.queries[:, :d])
+        base_d = np.ascontiguousarray(w.base[:, :d])
+        io, do = O.search(base_d, w.graph, q, w.pids, w.hits, po)
+        ig, dg = ctx.search(q, w.hits, make_params(k=10, beam=64))
+        assert_same(ig, io, f"ids d={d}")
+        assert_same(dg.view(np.int32), do.view(np.int32), f"dists d={d}")
+
+
+def test_select_entry_parity(ctx, small_workload, int_workload):
+    for w in (small_workload, int_workload):
+        ctx.set_leaves(w.aabbs, w.pids)
+        h = ctx.select_entry(w.queries)
+        assert_same(h, w.hits, "hits")
+        # points outside every box map to leaf 0; points on box faces are inside
+        far = np.full((4, w.queries.shape[1]), 1e9, np.float32)
+        assert (ctx.select_entry(far) == 0).all()
+        face = np.zeros((w.aabbs.shape[0], w.queries.shape[1]), np.float32)
+        face[:, :3] = w.aabbs[:, 3:6]
+        assert_same(ctx.select_entry(face), O.select_entry(w.aabbs, face), "faces")
+
+
+def test_projection_tolerance(ctx, small_workload):
+    w = small_workload
+    Q = w.raw_queries
+    d = Q.shape[1]
+    for d_out in (128, 64, 3):
+        ctx.set_pca(w.R, w.mu, d_out)
+        pq = ctx.project(Q)
+        ref = O.project_f64(Q, w.R, w.mu, d_out)
+        # FP32 GEMM bound: |err| <= c * d * eps32 * sum_j |q_j||R_jc| (+ the
+        # mu.R term); c = 4 covers the bias subtraction and the mu.R GEMV
+        bound = 4 * d * 2.0 ** -24 * (
+            np.abs(Q).astype(np.float64) @ np.abs(w.R[:, :d_out]).astype(np.float64)
+            + np.abs(w.mu).astype(np.float64) @ np.abs(w.R[:, :d_out]).astype(np.float64))
+        err = np.abs(pq.astype(np.float64) - ref)
+        assert (err <= bound + 1e-30).all(), f"max err {err.max()} d_out={d_out}"
+
+
+def test_project_then_select_then_search(ctx, int_workload):
+    """The whole path on device tensors, checked stage by stage."""
+    import torch
+    from rt_entry_amd import make_params
+    w = int_workload
+    dev = torch.device("cuda:0")
+    base = torch.from_numpy(w.base).to(dev)
+    graph = torch.from_numpy(w.graph_raw).to(dev)
+    ctx.set_base(base)
+    ctx.set_graph(graph)
+    ctx.set_leaves(w.aabbs, w.pids)
+    ctx.set_pca(w.R, w.mu, w.base.shape[1])
+    Q = torch.from_numpy(w.raw_queries).to(dev)
+    pq = ctx.project(Q)
+    hits = ctx.select_entry(pq)
+    ids, dists = ctx.search(pq, hits, make_params(k=10, beam=64))
+    ctx.synchronize()
+    pq_h = pq.cpu().numpy()
+    assert_same(hits.cpu().numpy(), O.select_entry(w.aabbs, pq_h), "hits")
+    io, do = O.search(w.base, w.graph, pq_h, w.pids, hits.cpu().numpy(), O.make_params(k=10, beam=64))
+    assert_same(ids.cpu().numpy(), io, "ids")
+    assert_same(dists.cpu().numpy().view(np.int32), do.view(np.int32), "dists")
+    assert O.recall(ids.cpu().numpy(), w.gt) > 0.8
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batched_uneven_parity(ctx, int_workload, mode):
+    from rt_entry_amd import data as D
+    from rt_entry_amd import make_params
+    w = int_workload
+    _ctx_for(ctx, w)
+    nq = w.queries.shape[0]
+    kq, bq = D.k_beam_assignment(3, nq)
+    po = O.make_params(k=10, beam=64, mode=mode)
+    if mode == 1:
+        po.bloom_words = 0
+    io, do = O.search_uneven(w.base, w.graph, w.queries, w.pids, w.hits, kq, bq, po)
+    ig, dg = ctx.search_batched(w.queries, w.hits, kq, bq, make_params(mode=mode),
+                                k_stride=io.shape[1])
+    for q in range(nq):
+        k = kq[q]
+        assert_same(ig[q, :k], io[q, :k], f"ids q={q} k={k} beam={bq[q]}")
+        assert_same(dg[q, :k].view(np.int32), do[q, :k].view(np.int32), f"dists q={q}")
+
+
+def test_errors_are_codes(ctx, small_workload):
+    from rt_entry_amd import RteError, make_params
+    w = small_workload
+    _ctx_for(ctx, w)
+    with pytest.raises(RteError):
+        ctx.search(w.queries, w.hits, make_params(k=100, beam=64))  # k > beam
+    with pytest.raises(RteError):
+        ctx.search(w.queries, w.hits, make_params(k=10, beam=64, width=9))
+    with pytest.raises(RteError):
+        ctx.search(w.queries[:, :3].copy(), w.hits, make_params(k=10, beam=64))  # q_stride < d
