@@ -1,0 +1,83 @@
+"""Workload preparation: the torch graph builder and kNN agree with the numpy
+ones (run here on CPU tensors), file formats round-trip."""
+import os
+
+import numpy as np
+import torch
+
+from conftest import assert_same
+
+
+def test_knn_and_graph_torch_match_numpy():
+    from rt_entry_amd import data as D
+    from rt_entry_amd import gpuprep as G
+    w = D.generate(2, n=3000, nq=50, n_train=0)
+    X, Q = w["base"], w["queries"]
+    kn, _ = D.knn_exact(X, X, 64, exclude_self=True)
+    kt, _ = G.knn_gpu(torch.from_numpy(X), None, 64, exclude_self=True, block=512)
+    assert_same(kt.numpy(), kn, "knn")
+    for deg, rev in ((32, 32), (32, 0), (48, 48)):
+        gn = D.graph_from_knn(kn, deg, rev_rank=rev)
+        gt = G.graph_from_knn_gpu(torch.from_numpy(kn), deg, rev_rank=rev)
+        assert_same(gt.numpy(), gn, f"graph deg={deg} rev={rev}")
+    gq, _ = D.knn_exact(X, Q, 10)
+    gq2, _ = G.knn_gpu(torch.from_numpy(X), torch.from_numpy(Q), 10, fp64=True)
+    assert_same(gq2.numpy(), gq, "gt")
+
+
+def test_graph_rules():
+    from rt_entry_amd import data as D
+    knn = np.array([[1, 2], [0, 2], [1, 0]], np.int32)
+    g = D.graph_from_knn(knn, 2, rev_rank=2)
+    # node 0: fwd 1 (r0), 2 (r1); reverse from 1 (r0), 2 (r1) -> dedupe
+    assert_same(g, np.array([[1, 2], [0, 2], [1, 0]], np.int32), "tiny graph")
+    assert (D.graph_from_knn(knn, 3, rev_rank=0)[:, 2] == 3).all()  # padded with n
+
+
+def test_pca_and_rotation():
+    from rt_entry_amd import data as D
+    from rt_entry_amd import gpuprep as G
+    w = D.generate(1, n=5000, nq=10, n_train=0)
+    R, mu, ev = D.pca_fit(w["base"])
+    assert (np.diff(ev) <= 1e-9).all()
+    assert np.allclose(R.T.astype(np.float64) @ R, np.eye(128), atol=1e-5)
+    R2, mu2 = G.pca_fit_gpu(torch.from_numpy(w["base"]))
+    assert np.allclose(np.abs((R2.T.astype(np.float64) @ R)).diagonal()[:20], 1.0, atol=1e-4)
+    x = D.pca_rotate(w["base"], R, mu)
+    # rotation preserves distances
+    a, b = w["base"][0].astype(np.float64), w["base"][1].astype(np.float64)
+    assert abs(np.sum((x[0] - x[1]).astype(np.float64) ** 2) - np.sum((a - b) ** 2)) < 1e-3 * np.sum((a - b) ** 2)
+
+
+def test_file_formats_roundtrip(tmp_path):
+    from rt_entry_amd import data as D
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(7, 5)).astype(np.float32)
+    i = rng.integers(0, 100, size=(7, 3)).astype(np.int32)
+    D.write_fvecs(tmp_path / "a.fvecs", x)
+    assert_same(D.read_fvecs(tmp_path / "a.fvecs"), x, "fvecs")
+    D.write_ivecs(tmp_path / "a.ivecs", i)
+    assert_same(D.read_ivecs(tmp_path / "a.ivecs"), i, "ivecs")
+    D.write_fbin(tmp_path / "a.fbin", x)
+    assert_same(D.read_fbin(tmp_path / "a.fbin"), x, "fbin")
+    D.write_nsw(tmp_path / "g.nsw", i)
+    assert_same(D.read_nsw(tmp_path / "g.nsw", 7), i, "nsw")
+    b = rng.normal(size=(4, 6)).astype(np.float32)
+    p = rng.integers(0, 9, size=(4, 8)).astype(np.int32)
+    D.write_aabbs(tmp_path / "x.aabbs", b, p)
+    b2, p2 = D.read_aabbs(tmp_path / "x.aabbs")
+    assert_same(b2, b, "aabbs")
+    assert_same(p2, p, "pids")
+    raw = np.zeros((3, 4 + 5), np.uint8)
+    raw[:, :4] = np.frombuffer(np.int32(5).tobytes(), np.uint8)
+    raw[:, 4:] = rng.integers(0, 255, size=(3, 5))
+    raw.tofile(tmp_path / "a.bvecs")
+    assert_same(D.read_bvecs(tmp_path / "a.bvecs"), raw[:, 4:].astype(np.float32), "bvecs")
+
+
+def test_k_beam_assignment():
+    from rt_entry_amd import data as D
+    k, b = D.k_beam_assignment(3, 1000)
+    assert set(np.unique(k)) == {1, 10, 100}
+    assert (b >= 32).all() and (b >= 128)[k == 100].all()
+    assert ((b & (b - 1)) == 0).all()
