@@ -66,7 +66,10 @@ def prepare(workload: int, scale: float, dev):
     gt = gt.cpu().numpy()
     log(f"ground truth {time.time() - t0:.1f}s")
     t0 = time.time()
-    cap = max(1, n // cfg["n_leaf"])
+    from rt_entry_amd import leaf_capacity
+    # entry.cu:22: aabb_size = np * point_ratio (float32); scaled runs keep
+    # the leaf count
+    cap = leaf_capacity(n, cfg["point_ratio"] / scale)
     pb_h = pbase.cpu().numpy()
     aabbs, boxes, pids = D.build_leaves(pb_h[:, :3], cap, 0.2)
     log(f"kd-tree {boxes.shape[0]} leaves cap {cap} in {time.time() - t0:.1f}s")
@@ -103,7 +106,9 @@ def main():
     nq = W["Q"].shape[0]
     k = 10
     ctx = Context(0)
-    stream = torch.cuda.current_stream()
+    # the query path runs on a stream of its own (default priority); HIP
+    # events bracket the search kernel on that stream
+    stream = torch.cuda.Stream(device=dev)
     ctx.set_stream(stream.cuda_stream)
     ctx.set_base(W["base"])
     ctx.set_graph(W["graph"])

@@ -51,7 +51,7 @@ typedef struct rte_params {
   int mode;         /* RTE_MODE_REF or RTE_MODE_TUNED */
   int algo;         /* 1: leaf entry table; 0 or 2: per-lane LCG entries */
   int bloom_words;  /* 0: default (REF 512, TUNED auto_tune_bloom sizing) */
-  int bloom_hashes; /* 0: default (REF 3, TUNED auto_tune_bloom sizing) */
+  int bloom_hashes; /* 0, or the mode's count: REF 3, TUNED 7 */
 } rte_params;
 
 /* ---- context ---------------------------------------------------------- */

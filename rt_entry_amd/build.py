@@ -23,7 +23,8 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librte.so")
 
-HIP_SOURCES = ["api.hip", "search.hip", "entry.hip", "project.hip", "mlp.hip"]
+HIP_SOURCES = ["api.hip", "search.hip", "search_nr1.hip", "search_nr2.hip", "search_nr4.hip",
+               "search_nr8.hip", "entry.hip", "project.hip", "mlp.hip"]
 CXX_SOURCES = ["kdtree.cpp"]
 
 HIPCC_FLAGS = [

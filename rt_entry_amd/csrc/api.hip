@@ -87,20 +87,20 @@ void hash_parameter(int nc, int* hash_len, int* bit, int* hash) {
 
 // Resolve the filter configuration for one beam.
 void bloom_for(const rte_params* p, int beam, int* words, int* hashes) {
+  // hash count per mode is fixed in the kernels: 3 (graph_search.cuh:5) or
+  // auto_tune_bloom.h's k (7 for every beam 32..512)
+  *hashes = p->mode == RTE_MODE_TUNED ? 7 : 3;
   if (p->bloom_words > 0) {
     *words = p->bloom_words;
-    *hashes = p->bloom_hashes > 0 ? p->bloom_hashes : 3;
     return;
   }
   if (p->mode == RTE_MODE_TUNED) {
     int hl, bit, hh;
     hash_parameter(beam, &hl, &bit, &hh);
     *words = hl;
-    *hashes = std::min(hh, 10);
     return;
   }
-  *words = 512;  // graph_search.cuh:4-5
-  *hashes = p->bloom_hashes > 0 ? p->bloom_hashes : 3;
+  *words = 512;  // graph_search.cuh:4
 }
 
 int check_params(rte_ctx* c, const rte_params* p, bool per_query) {
@@ -120,8 +120,9 @@ int check_params(rte_ctx* c, const rte_params* p, bool per_query) {
   if (p->bloom_words != 0 &&
       (!is_pow2(p->bloom_words) || p->bloom_words < 2 || p->bloom_words > 2048))
     return fail(c, RTE_EINVAL, "bloom_words must be a power of two in 2..2048");
-  if (p->bloom_hashes < 0 || p->bloom_hashes > 10)
-    return fail(c, RTE_EINVAL, "bloom_hashes must be in 0..10");
+  if (p->bloom_hashes != 0 && p->bloom_hashes != (p->mode == RTE_MODE_TUNED ? 7 : 3))
+    return fail(c, RTE_EUNSUP, "bloom_hashes must be 0 or %d in this mode",
+                p->mode == RTE_MODE_TUNED ? 7 : 3);
   return RTE_OK;
 }
 

@@ -40,9 +40,17 @@ struct SearchArgs {
   int auto_hashes[16];
   const int32_t* k_per_q;     // uneven batches (or null)
   const int32_t* beam_per_q;  // uneven batches (or null)
+  int64_t* work_counter;      // persistent launch: shared query counter (or null)
+  const int32_t* order;       // persistent launch: query order (or null)
 };
 
 int search_supported(int beam_max, int m, int d);
 hipError_t launch_search(const SearchArgs& a, int beam_max, int grid, hipStream_t st);
+// occupancy-derived persistent grid for a launch shape (workgroups)
+int search_persistent_grid(const SearchArgs& a, int beam_max);
+
+// per-beam-register-count launchers (one translation unit each)
+template <int NR, int MR>
+hipError_t launch_nm(const SearchArgs& a, int grid, hipStream_t st);
 
 }  // namespace rte

@@ -1,0 +1,735 @@
+// Greedy beam search on a proximity graph, one wave64 per query (gfx950).
+//
+// Restates GraphSearchKernel (graph_search.cuh:56-1964) for CDNA4:
+//  - one wavefront owns one query; a 256-thread workgroup carries 4
+//    independent queries and never uses a workgroup barrier;
+//  - the beam L (N slots) and the expansion batch B (M = width * 2^shift
+//    slots) live in VGPRs, slot s in lane s % 64 of register s / 64; the
+//    reference's compare-exchange networks (bitonic sort of B
+//    :888-919/:1424-1453, min-exchange :921-931/:1455-1467, half-cleaner
+//    merge of L :933-958/:1469-1493) run as cross-lane exchanges (DPP and
+//    gfx950 permlane swaps), so every pair sees exactly the comparison the
+//    reference makes;
+//  - the visited-set filter (:4-43) lives in LDS, one region per wave;
+//  - distances follow the reference association: each 32-lane half-wave
+//    owns one point, lane l accumulates dims l+32j as an FMA chain, and the
+//    lane partials are summed in the shfl_down tree 16,8,4,2,1 (:686-878),
+//    here as a transpose-reduce that yields 16 points per half-wave in 38
+//    VALU operations with the same pairings;
+//  - neighbour ids that pass the filter are compacted before the gathers, so
+//    one iteration issues its loads together and only for points that need a
+//    distance.
+// Parity decisions P1-P7 match the CPU oracle (zeroed filter, all-ones first
+// mask, snapshot filter test per row, no row read for ids >= np,
+// barrier-separated substeps, REF/TUNED comparators).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "search.h"
+
+namespace rte {
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// graph_search.cuh:89-97; h is a compile-time index after unrolling, so the
+// multipliers become literals
+__device__ __forceinline__ uint32_t hash_const(int i) {
+  constexpr uint32_t k[20] = {
+      0x924ed183U, 0xd854fc0aU, 0xecf5e3b7U, 0x1bead407U, 0x28a30449U,
+      0xbfc4d99fU, 0x715030e2U, 0xffcfb45bU, 0x6e4ce166U, 0xeb53c362U,
+      0xa93c4f40U, 0xcecde0a4U, 0x0288592dU, 0x362c37bcU, 0x9d4824f0U,
+      0xfdbdd68bU, 0x63258c85U, 0x6726905cU, 0x609500f9U, 0x4de48422U};
+  return k[i];
+}
+
+__device__ __forceinline__ uint32_t hash_raw(int h, uint32_t x) {  // :13-22
+  x ^= x >> 16;
+  x *= hash_const(2 * h);
+  x ^= x >> 13;
+  x *= hash_const(2 * h + 1);
+  x ^= x >> 16;
+  return x;
+}
+
+constexpr int kMaxHashes = 10;
+// hash count per comparator mode: the reference's 3 (graph_search.cuh:5) and
+// auto_tune_bloom.h's k, which is 7 for every beam from 32 to 512
+template <bool TUNED>
+constexpr int kHashes = TUNED ? 7 : 3;
+
+// The filter's hash loops are uniform (hashes is a scalar), and they are
+// always run by the whole wave: lanes that must not add OR in 0, so no
+// uniform loop ever sits inside a divergent region (where the compiler
+// would turn its guards into exec masks).
+template <int H>
+struct Bloom {
+  uint32_t* w;
+  uint32_t words;  // power of two
+  uint32_t lgw;    // log2(words)
+  uint32_t bits;   // words * 32
+
+  // bit positions of x, computed once for a test-then-add
+  __device__ __forceinline__ void positions(uint32_t x, uint32_t (&pos)[H]) const {
+#pragma unroll
+    for (int h = 0; h < H; ++h) pos[h] = hash_raw(h, x) & (bits - 1u);
+  }
+  __device__ __forceinline__ bool test_pos(const uint32_t (&pos)[H]) const {
+    bool ok = true;  // :25-28, :37-43
+#pragma unroll
+    for (int h = 0; h < H; ++h) ok &= ((w[pos[h] & (words - 1)] >> (pos[h] >> lgw)) & 1u) != 0;
+    return ok;
+  }
+  __device__ __forceinline__ void add_pos(const uint32_t (&pos)[H], bool en) const {
+#pragma unroll  // :8-11, :31-34
+    for (int h = 0; h < H; ++h)
+      atomicOr(&w[pos[h] & (words - 1)], en ? 1u << (pos[h] >> lgw) : 0u);
+  }
+  __device__ __forceinline__ void add(uint32_t x, bool en) const {
+    uint32_t pos[H];
+    positions(x, pos);
+    add_pos(pos, en);
+  }
+  __device__ __forceinline__ void clear(int lane) const {
+    for (int i = 0; i < (int)words; i += 64)
+      if (i + lane < (int)words) w[i + lane] = 0u;
+  }
+};
+
+// a > b under the configured comparator (P7)
+template <bool TUNED>
+__device__ __forceinline__ bool key_gt(float ad, int ai, float bd, int bi) {
+  if constexpr (!TUNED) {
+    return ad > bd;
+  } else {
+    if (ad != bd) return ad > bd;
+    return abs(ai) > abs(bi);
+  }
+}
+
+// Value of lane ^ S (S a power of two below 64) without LDS traffic:
+// quad_perm DPP for 1 and 2, row_half_mirror + quad_perm reverse for 4,
+// row_ror:8 for 8, permlane16_swap for 16 and permlane32_swap for 32.
+template <int S>
+__device__ __forceinline__ int xor_lane(int v, int lane) {
+  if constexpr (S == 1) {
+    return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+  } else if constexpr (S == 2) {
+    return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+  } else if constexpr (S == 4) {
+    // half_mirror: i -> 7-i inside each 8 lanes; then reverse each quad
+    const int m = __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true);
+    return __builtin_amdgcn_mov_dpp(m, 0x1B, 0xF, 0xF, true);  // quad_perm [3,2,1,0]
+  } else if constexpr (S == 8) {
+    return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, true);  // row_ror:8
+  } else if constexpr (S == 16) {
+    // rows r0..r3 of x=y=v: x' = [v0 v0 v2 v2], y' = [v1 v1 v3 v3]
+    const auto r = __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false);
+    return (int)((lane & 16) ? r[0] : r[1]);
+  } else {
+    static_assert(S == 32, "xor distance");
+    // halves of x=y=v: x' = [lo lo], y' = [hi hi]
+    const auto r = __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+    return (int)((lane & 32) ? r[0] : r[1]);
+  }
+}
+template <int S>
+__device__ __forceinline__ float xor_lane(float v, int lane) {
+  return __int_as_float(xor_lane<S>(__float_as_int(v), lane));
+}
+
+// Pair directions: a bitonic-sort step (graph_search.cuh:888-919: a pair is
+// descending when (e & 2*STEP) == 0) or the merge (all pairs ascending,
+// :933-958).
+template <int STEP>
+struct SortDir {
+  static constexpr bool desc(int e) { return (e & (2 * STEP)) == 0; }
+};
+struct MergeDir {
+  static constexpr bool desc(int) { return false; }
+};
+
+// Lanes of register r whose element must end a compare-exchange at
+// distance SUB holding the larger key: the lower element of a descending
+// pair or the upper element of an ascending one.
+template <int SUB, class Dir>
+constexpr uint64_t want_larger_mask(int r) {
+  uint64_t m = 0;
+  for (int l = 0; l < 64; ++l) {
+    const int e = r * 64 + l;
+    if (((e & SUB) == 0) == Dir::desc(e)) m |= 1ull << l;
+  }
+  return m;
+}
+
+// Lane mask of "the partner's key is strictly larger / smaller" under the
+// configured comparator (P7).
+template <bool TUNED>
+__device__ __forceinline__ void key_masks(float md, int mi, float pd, int pi, uint64_t& pgt,
+                                          uint64_t& plt) {
+  pgt = __builtin_amdgcn_ballot_w64(pd > md);
+  plt = __builtin_amdgcn_ballot_w64(pd < md);
+  if constexpr (TUNED) {
+    const uint64_t eq = __builtin_amdgcn_ballot_w64(pd == md);
+    const int ap = abs(pi), am = abs(mi);
+    pgt |= eq & __builtin_amdgcn_ballot_w64(ap > am);
+    plt |= eq & __builtin_amdgcn_ballot_w64(ap < am);
+  }
+}
+
+// One register of a cross-lane stage: the lane that must end with the
+// larger key takes its partner's value only when the partner is strictly
+// larger, the other lane only when it is strictly smaller -- exactly the
+// reference's swap on a strict compare.  The direction pattern is a
+// compile-time lane mask, so the choice is scalar-unit work.
+template <bool TUNED, int SUB, class Dir, int R, int r>
+__device__ __forceinline__ void stage_reg(float (&d)[R], int (&id)[R], int lane) {
+  if constexpr (r < R) {
+    constexpr uint64_t W = want_larger_mask<SUB, Dir>(r);
+    const float pd = xor_lane<SUB>(d[r], lane);
+    const int pi = xor_lane<SUB>(id[r], lane);
+    uint64_t pgt, plt;
+    key_masks<TUNED>(d[r], id[r], pd, pi, pgt, plt);
+    const bool take = __builtin_amdgcn_inverse_ballot_w64((pgt & W) | (plt & ~W));
+    d[r] = take ? pd : d[r];
+    id[r] = take ? pi : id[r];
+    stage_reg<TUNED, SUB, Dir, R, r + 1>(d, id, lane);
+  }
+}
+
+// Network stage over an R-register array (element e = r*64 + lane) at a
+// compile-time distance SUB.
+template <bool TUNED, int R, int SUB, class Dir>
+__device__ __forceinline__ void net_stage(float (&d)[R], int (&id)[R], int lane) {
+  if constexpr (SUB >= 64) {
+    constexpr int RS = SUB / 64;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if ((r & RS) == 0) {
+        // the direction depends on r only (2*STEP >= 128 here)
+        const bool dsc = Dir::desc(r * 64);
+        const float ld = d[r], hd = d[r + RS];
+        const int li = id[r], hi = id[r + RS];
+        const bool swap = dsc ? key_gt<TUNED>(hd, hi, ld, li) : key_gt<TUNED>(ld, li, hd, hi);
+        d[r] = swap ? hd : ld;
+        id[r] = swap ? hi : li;
+        d[r + RS] = swap ? ld : hd;
+        id[r + RS] = swap ? li : hi;
+      }
+    }
+  } else {
+    stage_reg<TUNED, SUB, Dir, R, 0>(d, id, lane);
+  }
+}
+
+// Bitonic sort, descending overall, of the first M_ elements (M_ a power of
+// two, M_ <= 64*R): graph_search.cuh:888-919.  Sizes are compile-time so a
+// network is straight-line code with no per-substep guards.
+template <bool TUNED, int R, int STEP, int SUB>
+__device__ __forceinline__ void sort_sub(float (&d)[R], int (&id)[R], int lane) {
+  net_stage<TUNED, R, SUB, SortDir<STEP>>(d, id, lane);
+  if constexpr (SUB > 1) sort_sub<TUNED, R, STEP, SUB / 2>(d, id, lane);
+}
+template <bool TUNED, int R, int M_, int STEP = 1>
+__device__ __forceinline__ void sort_net(float (&d)[R], int (&id)[R], int lane) {
+  if constexpr (STEP <= M_ / 2) {
+    sort_sub<TUNED, R, STEP, STEP>(d, id, lane);
+    sort_net<TUNED, R, M_, STEP * 2>(d, id, lane);
+  }
+}
+// runtime m (a uniform power of two) -> the compile-time network
+template <bool TUNED, int R, int M_ = 64 * R>
+__device__ __forceinline__ void sort_desc(float (&d)[R], int (&id)[R], int m, int lane) {
+  if constexpr (M_ >= 2) {
+    if (m == M_) sort_net<TUNED, R, M_>(d, id, lane);
+    else sort_desc<TUNED, R, M_ / 2>(d, id, m, lane);
+  }
+}
+
+// Half-cleaner cascade, every pair ascending, substeps N_/2..1:
+// graph_search.cuh:933-958.
+template <bool TUNED, int R, int SUB>
+__device__ __forceinline__ void merge_net(float (&d)[R], int (&id)[R], int lane) {
+  net_stage<TUNED, R, SUB, MergeDir>(d, id, lane);
+  if constexpr (SUB > 1) merge_net<TUNED, R, SUB / 2>(d, id, lane);
+}
+template <bool TUNED, int R, int N_ = 64 * R>
+__device__ __forceinline__ void merge_asc(float (&d)[R], int (&id)[R], int n, int lane) {
+  if constexpr (N_ >= 2) {
+    if (n == N_) merge_net<TUNED, R, N_ / 2>(d, id, lane);
+    else merge_asc<TUNED, R, N_ / 2>(d, id, n, lane);
+  }
+}
+
+// LDS layout of one wave's region (in 32-bit words)
+struct WaveLds {
+  uint32_t* bloom;  // bloom_words
+  int* cid;         // 64*MR compacted point ids
+  float* cdist;     // 64*MR compacted distances
+  float* sbd;       // 64*MR sorted batch distances
+  int* sbi;         // 64*MR sorted batch ids
+};
+
+template <int MR>
+__host__ __device__ constexpr int lds_words_extra() {
+  return 4 * 64 * MR;
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+// Sum of 16 per-point lane partials over the 32 lanes of each half-wave,
+// by recursive halving: at distance o the lane keeps the half of its points
+// selected by lane bit o and adds the partner's partial of those points.
+// Every partial combines lanes l and l^o exactly as the shfl_down tree
+// 16,8,4,2,1 does (graph_search.cuh:873-878), so each total is bit-identical
+// to the reference's; afterwards lanes 2p and 2p+1 of a half hold point p.
+__device__ __forceinline__ float transpose_reduce16(const float (&v)[16], int lane) {
+  float a[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    // x = v[i], y = v[i+8]: x' = [x0 y0 x2 y2], y' = [x1 y1 x3 y3], so
+    // x' + y' is point i in rows 0/2 and point i+8 in rows 1/3
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[i]),
+                                                    __float_as_uint(v[i + 8]), false, false);
+    a[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+  float b[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float lo = a[i] + dpp_f<0x108>(a[i]);          // row_shl:8, lane+8
+    const float hi = a[i + 4] + dpp_f<0x118>(a[i + 4]);  // row_shr:8, lane-8
+    b[i] = (lane & 8) ? hi : lo;
+  }
+  float c[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const float lo = b[i] + dpp_f<0x104>(b[i]);          // row_shl:4
+    const float hi = b[i + 2] + dpp_f<0x114>(b[i + 2]);  // row_shr:4
+    c[i] = (lane & 4) ? hi : lo;
+  }
+  const float lo = c[0] + dpp_f<0x4E>(c[0]);  // quad_perm xor 2
+  const float hi = c[1] + dpp_f<0x4E>(c[1]);
+  const float e = (lane & 2) ? hi : lo;
+  return e + dpp_f<0xB1>(e);  // quad_perm xor 1
+}
+
+// shfl_down tree of one point per half-wave; lane 0 of each half has it
+__device__ __forceinline__ float tree_reduce(float acc, int lane) {
+  acc += xor_lane<16>(acc, lane);
+  acc += xor_lane<8>(acc, lane);
+  acc += xor_lane<4>(acc, lane);
+  acc += xor_lane<2>(acc, lane);
+  acc += xor_lane<1>(acc, lane);
+  return acc;
+}
+
+template <bool TUNED, int NR, int MR, int J>
+struct SearchCore {
+  // points per half-wave in one load group: all 16 when the vectors fit in
+  // 64 VGPRs, so one iteration's gathers are issued together
+  static constexpr int PB = (64 / J) >= 16 ? 16 : ((64 / J) > 0 ? (64 / J) : 1);
+
+  // distances of the valid ids of B (id in [0,np)); others get MAXD.
+  // Returns the number of distances computed.
+  static __device__ __forceinline__ int batch_dists(const SearchArgs& a, const WaveLds& s,
+                                                    const float (&qv)[J], int lane,
+                                                    const int (&bi)[MR], float (&bd)[MR],
+                                                    int m) {
+    const int half = lane >> 5, l32 = lane & 31;
+    int total = 0;
+    int cix[MR];
+#pragma unroll
+    for (int r = 0; r < MR; ++r) {
+      const int slot = r * 64 + lane;
+      const bool valid = slot < m && bi[r] >= 0 && (int64_t)bi[r] < a.np;
+      const uint64_t bal = __ballot(valid);
+      const int ci = total + (int)__builtin_amdgcn_mbcnt_hi(
+                                 (uint32_t)(bal >> 32),
+                                 __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+      if (valid) s.cid[ci] = bi[r];
+      cix[r] = valid ? ci : -1;
+      total += __popcll(bal);
+    }
+    wave_sync();
+    const bool full = a.d == 32 * J;
+    const char* basep = reinterpret_cast<const char*>(a.base) + 4 * l32;
+    const int64_t rowb = a.stride * 4;
+    for (int p0 = 0; p0 < total; p0 += 2 * PB) {
+      float v[PB][J];
+#pragma unroll
+      for (int pb = 0; pb < PB; ++pb) {
+        // groups past the end reload the last point (finite, discarded)
+        const int kk = min(p0 + 2 * pb + half, total - 1);
+        const float* row = reinterpret_cast<const float*>(basep + (int64_t)s.cid[kk] * rowb);
+        if (full) {
+#pragma unroll
+          for (int j = 0; j < J; ++j) v[pb][j] = row[32 * j];
+        } else {
+#pragma unroll
+          for (int j = 0; j < J; ++j) v[pb][j] = (l32 + 32 * j) < a.d ? row[32 * j] : 0.0f;
+        }
+      }
+      float part[PB];
+#pragma unroll
+      for (int pb = 0; pb < PB; ++pb) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          const float t = v[pb][j] - qv[j];
+          acc = __builtin_fmaf(t, t, acc);
+        }
+        part[pb] = acc;
+      }
+      if constexpr (PB == 16) {
+        const float tot = transpose_reduce16(part, lane);
+        const int kk = p0 + 2 * (l32 >> 1) + half;
+        if ((lane & 1) == 0 && kk < total) s.cdist[kk] = tot;
+      } else {
+#pragma unroll
+        for (int pb = 0; pb < PB; ++pb) {
+          const float tot = tree_reduce(part[pb], lane);
+          const int kk = p0 + 2 * pb + half;
+          if (l32 == 0 && kk < total) s.cdist[kk] = tot;
+        }
+      }
+    }
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < MR; ++r) bd[r] = cix[r] >= 0 ? s.cdist[cix[r]] : kMaxD;
+    wave_sync();
+    return total;
+  }
+
+  // sort B descending, min-exchange into L, merge L ascending
+  static __device__ __forceinline__ void merge_batch(const WaveLds& s, int lane, float (&ld)[NR],
+                                                     int (&li)[NR], float (&bd)[MR],
+                                                     int (&bi)[MR], int n, int m,
+                                                     bool skip_max) {
+    sort_desc<TUNED, MR>(bd, bi, m, lane);
+#pragma unroll
+    for (int r = 0; r < MR; ++r) {
+      s.sbd[r * 64 + lane] = bd[r];
+      s.sbi[r * 64 + lane] = bi[r];
+    }
+    wave_sync();
+    const int c = n < m ? n : m;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int sl = r * 64 + lane;
+      if (sl >= n - c && sl < n) {
+        const int sb = sl - n + m;
+        const float xd = s.sbd[sb];
+        const int xi = s.sbi[sb];
+        if (!(skip_max && xd == kMaxD) && key_gt<TUNED>(ld[r], li[r], xd, xi)) {
+          ld[r] = xd;
+          li[r] = xi;
+        }
+      }
+    }
+    wave_sync();
+    merge_asc<TUNED, NR>(ld, li, n, lane);
+  }
+};
+
+// Expansion-flag ballots of the beam (graph_search.cuh:1014-1031).
+template <int NR>
+__device__ __forceinline__ void flag_ballots(const int (&li)[NR], int lane, int n, int64_t np,
+                                             uint64_t (&bf)[NR], uint64_t (&bz)[NR]) {
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int sl = r * 64 + lane;
+    const bool in = sl < n;
+    bf[r] = __ballot(in && li[r] > 0 && (int64_t)li[r] < np);
+    bz[r] = __ballot(in && li[r] == 0);
+  }
+}
+
+template <int NR>
+__device__ __forceinline__ uint32_t scan_chunk(const uint64_t (&bf)[NR], const uint64_t (&bz)[NR],
+                                               int chunk, uint32_t& czero) {
+  uint32_t vf = 0, vz = 0;
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+    if ((chunk >> 1) == r) {
+      vf = (chunk & 1) ? (uint32_t)(bf[r] >> 32) : (uint32_t)bf[r];
+      vz = (chunk & 1) ? (uint32_t)(bz[r] >> 32) : (uint32_t)bz[r];
+    }
+  const uint32_t zm = vz & ~czero;
+  czero |= zm;
+  return vf | zm;
+}
+
+// next query of this wave: a shared work counter (persistent launch) or a
+// grid stride
+__device__ __forceinline__ int64_t next_query(const SearchArgs& a, int64_t prev, int wave,
+                                              int lane) {
+  if (a.work_counter) {
+    int64_t i = 0;
+    if (lane == 0) i = atomicAdd(reinterpret_cast<unsigned long long*>(a.work_counter), 1ull);
+    i = __builtin_amdgcn_readfirstlane((int)i);
+    if (i >= a.nq) return a.nq;
+    return a.order ? a.order[i] : i;
+  }
+  return prev < 0 ? (int64_t)blockIdx.x * 4 + wave : prev + (int64_t)gridDim.x * 4;
+}
+
+template <bool TUNED, int NR, int MR, int J>
+__global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
+  using Core = SearchCore<TUNED, NR, MR, J>;
+  extern __shared__ uint32_t smem[];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int region = a.bloom_words_max + lds_words_extra<MR>();
+  uint32_t* base_lds = smem + wave * region;
+  const int shift = a.shift;
+  const int D = 1 << shift;
+  const int W = a.width;
+  const int m = W << shift;
+  const int64_t np = a.np;
+  const int32_t npi = (int32_t)np;
+
+  for (int64_t q = next_query(a, -1, wave, lane); q < a.nq; q = next_query(a, q, wave, lane)) {
+    // per-query shape (uneven batches carry their own k and beam)
+    int n = a.beam, k = a.k;
+    if (a.beam_per_q) {
+      n = a.beam_per_q[q];
+      k = a.k_per_q[q];
+    }
+    int bwords = a.bloom_words, bhash = a.bloom_hashes;
+    if (a.bloom_auto) {  // auto_tune_bloom sizing per beam
+      const int lg = 31 - __clz(n);
+      bwords = a.auto_words[lg];
+      bhash = a.auto_hashes[lg];
+    }
+    // per-query shapes are wave-uniform: keep them in SGPRs so loops over
+    // them are scalar branches, not exec-masked ones
+    n = __builtin_amdgcn_readfirstlane(n);
+    k = __builtin_amdgcn_readfirstlane(k);
+    bwords = __builtin_amdgcn_readfirstlane(bwords);
+    bhash = __builtin_amdgcn_readfirstlane(bhash);
+    WaveLds s;
+    s.bloom = base_lds;
+    s.cid = (int*)(base_lds + a.bloom_words_max);
+    s.cdist = (float*)(s.cid + 64 * MR);
+    s.sbd = s.cdist + 64 * MR;
+    s.sbi = (int*)(s.sbd + 64 * MR);
+    (void)bhash;  // fixed per mode (kHashes), checked on the host
+    Bloom<kHashes<TUNED>> bl{s.bloom, (uint32_t)bwords, (uint32_t)(31 - __clz(bwords)),
+                             (uint32_t)bwords * 32u};
+
+    float qv[J];
+    const float* qp = a.queries + q * a.q_stride;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int dim = (lane & 31) + 32 * j;
+      qv[j] = dim < a.d ? qp[dim] : 0.0f;
+    }
+    bl.clear(lane);  // P1
+    wave_sync();
+
+    float ld[NR], bd[MR];
+    int li[NR], bi[MR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      ld[r] = kMaxD;
+      li[r] = npi;
+    }
+    int hit = 0;
+    if (a.algo == 1) {
+      hit = __builtin_amdgcn_readfirstlane(a.hits[q]);
+      if (hit < 0 || hit >= a.n_leaf) hit = 0;
+    }
+    const int32_t* prow = a.pids + (int64_t)hit * a.cap;
+    uint32_t seed = (uint32_t)(q * 64 + lane);  // :66
+    int nE = 0, nX = 0;
+
+    // ---- entry seeding, :479-959
+    const int nch = (n + m - 1) / m;
+    for (int c = 0; c < nch; ++c) {
+#pragma unroll
+      for (int r = 0; r < MR; ++r) {
+        const int i = r * 64 + lane;
+        const int idx = c * m + i;
+        int id = npi;
+        const bool in = i < m && idx < n;
+        if (in) {
+          if (a.algo == 1) {
+            id = idx < a.cap ? prow[idx] : npi;
+          } else {  // :50-54
+            seed = seed * 1103515245u + 12345u;
+            id = (int)(seed % (uint32_t)np);
+          }
+        }
+        bl.add((uint32_t)id, in);
+        bi[r] = id;
+      }
+      wave_sync();
+      nE += Core::batch_dists(a, s, qv, lane, bi, bd, m);
+      Core::merge_batch(s, lane, ld, li, bd, bi, n, m, false);
+    }
+
+    // ---- main loop, :964-1532
+    int iter = 0, chunk = 0, hash_it = 0;
+    bool done = false;
+    uint32_t czero = 0;
+    uint32_t mask = n >= 32 ? 0xffffffffu : ((1u << n) - 1u);  // P2
+    const int nchunks = (n + 31) >> 5;
+    while (!done && iter < a.max_iter) {
+      ++iter;
+#pragma unroll
+      for (int r = 0; r < MR; ++r) bi[r] = npi;
+      // pick up to W slots (:985-1031)
+      int cnt = 0;
+      int vrow[kMaxWidth];
+#pragma unroll
+      for (int x = 0; x < kMaxWidth; ++x) vrow[x] = 0;
+      while (cnt < W && mask != 0u) {
+        const int b = __builtin_ctz(mask);
+        const int sl = chunk * 32 + b;
+        int v = 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+          if ((sl >> 6) == r) {
+            v = __builtin_amdgcn_readlane(li[r], sl & 63);
+            if (lane == (sl & 63)) li[r] = -li[r];
+          }
+#pragma unroll
+        for (int x = 0; x < kMaxWidth; ++x)
+          if (x == cnt) vrow[x] = v < 0 ? -v : v;
+        mask &= ~(1u << b);
+        ++cnt;
+        if (cnt == W) break;
+        while (mask == 0u && chunk < nchunks) {
+          ++chunk;
+          uint64_t bf[NR], bz[NR];
+          flag_ballots<NR>(li, lane, n, np, bf, bz);
+          mask = chunk < nchunks ? scan_chunk<NR>(bf, bz, chunk, czero) : 0u;
+        }
+      }
+      if (cnt == 0) break;
+      // graph rows into B (rows past np are not read, P4)
+#pragma unroll
+      for (int r = 0; r < MR; ++r) {
+        const int i = r * 64 + lane;
+        const int w = i >> shift;
+        if (i < m && w < cnt) {
+          int vv = 0;
+#pragma unroll
+          for (int x = 0; x < kMaxWidth; ++x)
+            if (x == w) vv = vrow[x];
+          bi[r] = (int64_t)vv < np ? a.graph[((int64_t)vv << shift) + (i & (D - 1))] : npi;
+        }
+      }
+      // filter test per row against the filter as it was before that row's
+      // own adds (P3), then the adds
+#pragma unroll
+      for (int w = 0; w < kMaxWidth; ++w) {
+        if (w >= cnt) break;
+        nX += (int64_t)vrow[w] < np ? 1 : 0;
+        uint32_t pos[MR][kHashes<TUNED>];
+        bool pass[MR];
+#pragma unroll
+        for (int r = 0; r < MR; ++r) {
+          const int i = r * 64 + lane;
+          const bool mine = i < m && (i >> shift) == w;
+          const int t = bi[r];
+          bl.positions((uint32_t)t, pos[r]);
+          const bool seen = bl.test_pos(pos[r]);
+          pass[r] = mine && t != npi && !seen;
+        }
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < MR; ++r) {
+          const int i = r * 64 + lane;
+          const bool mine = i < m && (i >> shift) == w;
+          bl.add_pos(pos[r], pass[r]);
+          bi[r] = (mine && !pass[r]) ? npi : bi[r];
+        }
+        wave_sync();
+      }
+      nE += Core::batch_dists(a, s, qv, lane, bi, bd, cnt << shift);
+      Core::merge_batch(s, lane, ld, li, bd, bi, n, m, true);
+      // rescan (:1495-1516)
+      {
+        uint64_t bf[NR], bz[NR];
+        flag_ballots<NR>(li, lane, n, np, bf, bz);
+        for (chunk = 0; chunk < nchunks; ++chunk) {
+          mask = scan_chunk<NR>(bf, bz, chunk, czero);
+          if (mask != 0u) break;
+          if (chunk == nchunks - 1) done = true;
+        }
+      }
+      if (hash_it == 4) {  // :1518-1531
+        bl.clear(lane);
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const int sl = r * 64 + lane;
+          const int x = li[r] < 0 ? -li[r] : li[r];
+          bl.add((uint32_t)x, sl < n && (int64_t)x < np);
+        }
+        wave_sync();
+        hash_it = 0;
+      }
+      ++hash_it;
+    }
+
+    // ---- output (:1955-1963), plus distances
+    const int64_t ob = q * (int64_t)a.k_stride;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int sl = r * 64 + lane;
+      if (sl < k) {
+        a.ids_out[ob + sl] = li[r] < 0 ? -li[r] : li[r];
+        if (a.dists_out) a.dists_out[ob + sl] = ld[r];
+      }
+    }
+    if (a.stats_out && lane == 0) {
+      a.stats_out[q * 3 + 0] = nE;
+      a.stats_out[q * 3 + 1] = nX;
+      a.stats_out[q * 3 + 2] = iter;
+    }
+    wave_sync();
+  }
+}
+
+template <bool TUNED, int NR, int MR, int J>
+hipError_t launch_t(const SearchArgs& a, int grid, hipStream_t st) {
+  const size_t lds = (size_t)4 * (a.bloom_words_max + lds_words_extra<MR>()) * 4;
+  hipLaunchKernelGGL((search_kernel<TUNED, NR, MR, J>), dim3(grid), dim3(256), lds, st, a);
+  return hipGetLastError();
+}
+
+template <int NR, int MR>
+hipError_t launch_nm(const SearchArgs& a, int grid, hipStream_t st) {
+  const int J = (a.d + 31) / 32;
+  const bool tuned = a.mode != 0;
+#define RTE_J_CASES(T)                                  \
+  switch (J) {                                          \
+    case 1:                                             \
+    case 2: return launch_t<T, NR, MR, 2>(a, grid, st); \
+    case 3: return launch_t<T, NR, MR, 3>(a, grid, st); \
+    case 4: return launch_t<T, NR, MR, 4>(a, grid, st); \
+    case 5:                                             \
+    case 6:                                             \
+    case 7:                                             \
+    case 8: return launch_t<T, NR, MR, 8>(a, grid, st); \
+    case 30: return launch_t<T, NR, MR, 30>(a, grid, st); \
+    default: return hipErrorInvalidValue;               \
+  }
+  if (tuned) {
+    RTE_J_CASES(true)
+  } else {
+    RTE_J_CASES(false)
+  }
+#undef RTE_J_CASES
+}
+
+}  // namespace rte

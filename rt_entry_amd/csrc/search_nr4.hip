@@ -1,0 +1,8 @@
+// Search kernel instantiations for beams up to 256 (one translation unit per
+// beam register count, so the build compiles them in parallel).
+#include "search_impl.h"
+
+namespace rte {
+template hipError_t launch_nm<4, 1>(const SearchArgs&, int, hipStream_t);
+template hipError_t launch_nm<4, 2>(const SearchArgs&, int, hipStream_t);
+}  // namespace rte

@@ -25,12 +25,16 @@ import numpy as np
 SEED0 = 20261202
 
 CONFIGS = {
-    1: dict(n=100_000, nq=1_000, d=128, degree=32, k_fwd=32, n_leaf=1024, k=10, beam=64),
-    2: dict(n=1_000_000, nq=10_000, d=128, degree=32, k_fwd=64, n_leaf=4096, k=10, beam=64),
-    3: dict(n=1_000_000, nq=20_000, d=96, degree=32, k_fwd=64, n_leaf=4096, k=10, beam=64),
-    4: dict(n=1_000_000, nq=10_000, d=960, degree=48, k_fwd=64, n_leaf=4096, k=10, beam=64,
-            n_train=100_000),
-    5: dict(n=10_000_000, nq=100_000, d=128, degree=32, k_fwd=64, n_leaf=16384, k=10, beam=64),
+    1: dict(n=100_000, nq=1_000, d=128, degree=32, k_fwd=32, n_leaf=1024, point_ratio=0.00128,
+            k=10, beam=64),
+    2: dict(n=1_000_000, nq=10_000, d=128, degree=32, k_fwd=64, n_leaf=4096,
+            point_ratio=0.000256, k=10, beam=64),
+    3: dict(n=1_000_000, nq=20_000, d=96, degree=32, k_fwd=64, n_leaf=4096,
+            point_ratio=0.000256, k=10, beam=64),
+    4: dict(n=1_000_000, nq=10_000, d=960, degree=48, k_fwd=64, n_leaf=4096,
+            point_ratio=0.000256, k=10, beam=64, n_train=100_000),
+    5: dict(n=10_000_000, nq=100_000, d=128, degree=32, k_fwd=64, n_leaf=16384,
+            point_ratio=0.0001024, k=10, beam=64),
 }
 
 

@@ -84,7 +84,8 @@ def test_search_parity_max_iter_and_bloom(ctx, small_workload):
     w = small_workload
     _ctx_for(ctx, w)
     for kw in (dict(max_iter=0), dict(max_iter=1), dict(max_iter=7),
-               dict(bloom_words=64, bloom_hashes=2), dict(bloom_words=2048, bloom_hashes=5)):
+               dict(bloom_words=64, bloom_hashes=3), dict(bloom_words=2048),
+               dict(bloom_words=32, mode=1), dict(bloom_words=1024, mode=1, bloom_hashes=7)):
         print(f"[parity] {kw}", flush=True)
         (io, do, _), (ig, dg, _) = _run_both(ctx, w, w.queries, w.hits, k=10, beam=64, **kw)
         assert_same(ig, io, f"ids {kw}")
@@ -194,3 +195,5 @@ def test_errors_are_codes(ctx, small_workload):
         ctx.search(w.queries, w.hits, make_params(k=10, beam=64, width=9))
     with pytest.raises(RteError):
         ctx.search(w.queries[:, :3].copy(), w.hits, make_params(k=10, beam=64))  # q_stride < d
+    with pytest.raises(RteError):  # hash count is fixed per mode
+        ctx.search(w.queries, w.hits, make_params(k=10, beam=64, bloom_hashes=5))
