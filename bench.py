@@ -37,7 +37,7 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def prepare(workload: int, scale: float, dev):
+def prepare(workload: int, scale: float, dev, n_train: int = 0):
     import torch
 
     import oracle as O
@@ -73,8 +73,48 @@ def prepare(workload: int, scale: float, dev):
     pb_h = pbase.cpu().numpy()
     aabbs, boxes, pids = D.build_leaves(pb_h[:, :3], cap, 0.2)
     log(f"kd-tree {boxes.shape[0]} leaves cap {cap} in {time.time() - t0:.1f}s")
-    return dict(base=pbase, base_h=pb_h, graph=graph, Q=Q, R=R, mu=mu, gt=gt, aabbs=aabbs,
-                pids=pids, n=n, d=d, cap=cap, n_leaf=boxes.shape[0], cfg=cfg)
+    out = dict(base=pbase, base_h=pb_h, graph=graph, Q=Q, R=R, mu=mu, gt=gt, aabbs=aabbs,
+               pids=pids, n=n, d=d, cap=cap, n_leaf=boxes.shape[0], cfg=cfg)
+    if n_train:
+        t0 = time.time()
+        tr = torch.from_numpy(D.extra_queries(workload, n_train)).to(dev)
+        out["train_raw"] = tr
+        out["gt_train"] = G.knn_gpu(base, tr, 10)[0].cpu().numpy()
+        log(f"{n_train} training queries + exact 10-NN in {time.time() - t0:.1f}s")
+    del base
+    return out
+
+
+def sweep_and_time(step, recall_of, beams, iters, mode, k, log_name):
+    """Cheapest (beam, max_iter) with recall@10 >= 0.95 (2 timed passes per
+    point; for each beam the first budget that reaches the target)."""
+    import torch
+    from rt_entry_amd import make_params
+
+    def time_steps(p, n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            step(p)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n
+
+    sweep, best = [], None
+    for beam in beams:
+        for mi in iters:
+            p = make_params(k=k, beam=beam, max_iter=mi, mode=mode)
+            rec = recall_of(p)
+            t = time_steps(p, 2)
+            sweep.append(dict(beam=beam, max_iter=mi, recall=round(rec, 4), ms=round(t * 1e3, 3)))
+            log(f"{log_name} sweep beam={beam} max_iter={mi} recall={rec:.4f} {t * 1e3:.3f} ms")
+            if rec >= 0.95:
+                if best is None or t < best[0]:
+                    best = (t, beam, mi, rec)
+                break
+    if best is None:
+        s = max(sweep, key=lambda x: x["recall"])
+        best = (s["ms"] / 1e3, s["beam"], s["max_iter"], s["recall"])
+    return best, sweep
 
 
 def main():
@@ -84,11 +124,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", type=int, default=2)
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--beam", type=int, default=0, help="skip the sweep: fixed beam")
+    ap.add_argument("--beam", type=int, default=0, help="skip the sweeps: fixed beam")
     ap.add_argument("--max-iter", type=int, default=0)
     ap.add_argument("--mode", type=int, default=0, help="0 REF comparator, 1 TUNED")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--n-train", type=int, default=100_000, help="0 skips the learned entry")
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--min-conf", type=float, default=0.5)
+    ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -100,9 +144,10 @@ def main():
 
     import oracle as O
     from rt_entry_amd import Context, make_params
+    from rt_entry_amd import predictor as P
 
     dev = torch.device("cuda:0")
-    W = prepare(args.workload, args.scale, dev)
+    W = prepare(args.workload, args.scale, dev, args.n_train)
     nq = W["Q"].shape[0]
     k = 10
     ctx = Context(0)
@@ -110,6 +155,9 @@ def main():
     # events bracket the search kernel on that stream
     stream = torch.cuda.Stream(device=dev)
     ctx.set_stream(stream.cuda_stream)
+    # torch work of the benchmark (gathers, label scoring, recall) runs on the
+    # same stream, so it is ordered with the library's kernels
+    torch.cuda.set_stream(stream)
     ctx.set_base(W["base"])
     ctx.set_graph(W["graph"])
     ctx.set_leaves(W["aabbs"], W["pids"])
@@ -117,136 +165,180 @@ def main():
 
     pq = torch.empty((nq, W["d"]), dtype=torch.float32, device=dev)
     hits = torch.empty(nq, dtype=torch.int32, device=dev)
+    learned = torch.empty(nq, dtype=torch.int32, device=dev)
     ids = torch.empty((nq, k), dtype=torch.int32, device=dev)
     dists = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    use_mlp = [False]
 
     def step(p):
         ctx.project(W["Q"], out=pq)
         ctx.select_entry(pq, out=hits)
-        ctx.search(pq, hits, p, out=(ids, dists, None))
+        entry = hits
+        if use_mlp[0]:
+            ctx.mlp_predict(pq, fallback=hits, min_conf=args.min_conf, out=learned)
+            entry = learned
+        ctx.search(pq, entry, p, out=(ids, dists, None))
 
     def recall_of(p):
         step(p)
         torch.cuda.synchronize()
         return O.recall(ids.cpu().numpy(), W["gt"], 10)
 
-    def time_steps(p, n):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(n):
-            step(p)
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / n
+    beams = (args.beam,) if args.beam else (32, 64, 128, 256)
+    iters = (args.max_iter or 100,) if args.beam else (10, 15, 20, 25, 30, 40, 50, 60, 80, 100,
+                                                       150, 200)
 
-    # ---- sweep: cheapest (beam, max_iter) with recall >= 0.95
-    if args.beam:
-        chosen = (args.beam, args.max_iter or 100)
-        rec = recall_of(make_params(k=k, beam=chosen[0], max_iter=chosen[1], mode=args.mode))
-        sweep = [dict(beam=chosen[0], max_iter=chosen[1], recall=rec)]
-    else:
-        sweep = []
-        best = None
-        for beam in (32, 64, 128, 256):
-            for mi in (20, 30, 40, 60, 80, 100, 150, 200):
-                if mi < beam // 4:
-                    continue
-                p = make_params(k=k, beam=beam, max_iter=mi, mode=args.mode)
-                rec = recall_of(p)
-                t = time_steps(p, 2)
-                sweep.append(dict(beam=beam, max_iter=mi, recall=round(rec, 4),
-                                  ms=round(t * 1e3, 3)))
-                log(f"sweep beam={beam} max_iter={mi} recall={rec:.4f} {t * 1e3:.3f} ms")
-                if rec >= 0.95:
-                    if best is None or t < best[0]:
-                        best = (t, beam, mi, rec)
-                    break
-        if best is None:
-            best = min(((s["ms"], s["beam"], s["max_iter"], s["recall"]) for s in sweep),
-                       key=lambda x: -x[3])
-        chosen = (best[1], best[2])
-        rec = best[3]
-    p = make_params(k=k, beam=chosen[0], max_iter=chosen[1], mode=args.mode)
-    log(f"chosen beam={chosen[0]} max_iter={chosen[1]} recall={rec:.4f}")
+    # ---- plain method: kd-tree entry
+    best_kd, sweep_kd = sweep_and_time(step, recall_of, beams, iters, args.mode, k, "kd-tree")
 
-    # ---- timed region
-    for _ in range(args.warmup):
-        step(p)
-    torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for i in range(args.steps):
+    # ---- learned entry: labels from the plain search, GPU training
+    best_ln, sweep_ln, pred_info = None, None, None
+    if args.n_train:
+        t0 = time.time()
+        PQt = ctx.project(W["train_raw"])
+        kd_t = ctx.select_entry(PQt)
+        torch.cuda.synchronize()
+        kd_h = kd_t.cpu().numpy()
+        labels, _ = P.make_labels(ctx, PQt, None, W["gt_train"], beam=64, mode=args.mode,
+                                  kd=kd_h, n_leaf=W["n_leaf"])
+        t_lab = time.time() - t0
+        log(f"labels in {t_lab:.1f}s; {(labels != kd_h).mean():.3f} differ from the kd leaf")
+        t0 = time.time()
+        losses = P.train(ctx, PQt, torch.from_numpy(labels).to(dev), W["n_leaf"],
+                         epochs=args.epochs, batch=1024, lr=2e-3,
+                         log=lambda m: log("train " + m) if args.verbose else None)
+        torch.cuda.synchronize()
+        t_train = time.time() - t0
+        log(f"trained {args.epochs} epochs in {t_train:.1f}s, loss {losses[-1]:.3f}")
+        use_mlp[0] = True
+        conf = torch.empty(nq, dtype=torch.float32, device=dev)
         ctx.project(W["Q"], out=pq)
         ctx.select_entry(pq, out=hits)
-        ev[i][0].record(stream)
-        ctx.search(pq, hits, p, out=(ids, dists, None))
-        ev[i][1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall / args.steps * 1e3
-    qps = nq * args.steps / wall
-    search_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-    final_recall = O.recall(ids.cpu().numpy(), W["gt"], 10)
+        ctx.mlp_predict(pq, fallback=hits, min_conf=args.min_conf, out=learned, conf=conf)
+        torch.cuda.synchronize()
+        c = conf.cpu().numpy()
+        pred_info = {"train_queries": args.n_train, "epochs": args.epochs,
+                     "final_loss": round(losses[-1], 4), "label_s": round(t_lab, 1),
+                     "train_s": round(t_train, 1), "min_conf": args.min_conf,
+                     "fallback_frac": round(float((c < args.min_conf).mean()), 4),
+                     "labels_not_kd_leaf": round(float((labels != kd_h).mean()), 4)}
+        best_ln, sweep_ln = sweep_and_time(step, recall_of, beams, iters, args.mode, k,
+                                           "learned")
 
-    # ---- algorithmic bytes of the search kernel (per query:
-    #      4d(E+1) + 4D X + 4N + 8k; E distance evaluations, X rows read)
-    stats = torch.empty((nq, 3), dtype=torch.int32, device=dev)
-    ctx.search(pq, hits, p, out=(ids, dists, stats))
-    torch.cuda.synchronize()
-    st = stats.cpu().numpy().astype(np.int64)
-    D = 1 << (W["cfg"]["degree"] - 1).bit_length()
-    alg_bytes = float((4 * W["d"] * (st[:, 0] + 1) + 4 * D * st[:, 1] + 4 * chosen[0]
-                       + 8 * k).sum())
-    achieved = alg_bytes / (search_ms * 1e-3)
+    def measure(use, best):
+        use_mlp[0] = use
+        p = make_params(k=k, beam=best[1], max_iter=best[2], mode=args.mode)
+        for _ in range(args.warmup):
+            step(p)
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            ctx.project(W["Q"], out=pq)
+            ctx.select_entry(pq, out=hits)
+            entry = hits
+            if use:
+                ctx.mlp_predict(pq, fallback=hits, min_conf=args.min_conf, out=learned)
+                entry = learned
+            ev[i][0].record(stream)
+            ctx.search(pq, entry, p, out=(ids, dists, None))
+            ev[i][1].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        rec = O.recall(ids.cpu().numpy(), W["gt"], 10)
+        search_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        pred_ms = None
+        if use:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(args.steps):
+                ctx.mlp_predict(pq, fallback=hits, min_conf=args.min_conf, out=learned)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            pred_ms = e0.elapsed_time(e1) / args.steps
+        stats = torch.empty((nq, 3), dtype=torch.int32, device=dev)
+        ctx.search(pq, entry, p, out=(ids, dists, stats))
+        torch.cuda.synchronize()
+        st = stats.cpu().numpy().astype(np.int64)
+        D = 1 << (W["cfg"]["degree"] - 1).bit_length()
+        alg = float((4 * W["d"] * (st[:, 0] + 1) + 4 * D * st[:, 1] + 4 * best[1] + 8 * k).sum())
+        return dict(qps=nq * args.steps / wall, ms=wall / args.steps * 1e3, recall=rec,
+                    search_ms=search_ms, alg_bytes=alg, p=p, entry=entry, beam=best[1],
+                    max_iter=best[2], iters=float(st[:, 2].mean()), pred_ms=pred_ms)
+
+    kd = measure(False, best_kd)
+    log(f"kd-tree entry: {kd['qps']:.0f} QPS recall {kd['recall']:.4f}")
+    ln = None
+    if best_ln is not None:
+        ln = measure(True, best_ln)
+        log(f"learned entry: {ln['qps']:.0f} QPS recall {ln['recall']:.4f}")
+    head = ln if (ln is not None and ln["recall"] >= 0.95) else kd
+
     traffic = None
     tf = os.path.join(ROOT, "profiles", "search_traffic.json")
     if os.path.exists(tf):
         try:
             tj = json.load(open(tf))
-            if tj.get("beam") == chosen[0] and tj.get("max_iter") == chosen[1]:
+            if tj.get("beam") == head["beam"] and tj.get("max_iter") == head["max_iter"]:
                 traffic = tj.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
+    achieved = head["alg_bytes"] / (head["search_ms"] * 1e-3)
 
     # ---- CPU baseline: the oracle's search on the host cores, bounded sample
     cpu = None
     if not args.no_cpu_baseline:
+        use_mlp[0] = head is ln
+        ctx.project(W["Q"], out=pq)
+        ctx.select_entry(pq, out=hits)
+        if use_mlp[0]:
+            ctx.mlp_predict(pq, fallback=hits, min_conf=args.min_conf, out=learned)
+        torch.cuda.synchronize()
         pq_h = pq.cpu().numpy()
-        hits_h = hits.cpu().numpy()
+        ent_h = (learned if use_mlp[0] else hits).cpu().numpy()
         g_h = W["graph"].cpu().numpy()
         from rt_entry_amd.data import pad_graph
         g_h = pad_graph(g_h, W["n"])
-        po = O.make_params(k=k, beam=chosen[0], max_iter=chosen[1], mode=args.mode)
+        po = O.make_params(k=k, beam=head["beam"], max_iter=head["max_iter"], mode=args.mode)
         threads = min(os.cpu_count() or 1, 16)
         m = 256
         t = 0.0
         while True:
             t0 = time.perf_counter()
-            O.search(W["base_h"], g_h, pq_h[:m], W["pids"], hits_h[:m], po, n_threads=threads)
+            O.search(W["base_h"], g_h, pq_h[:m], W["pids"], ent_h[:m], po, n_threads=threads)
             t = time.perf_counter() - t0
             if t >= args.cpu_seconds / 4 or m >= nq:
                 break
             m = min(nq, int(m * max(2.0, (args.cpu_seconds / 4) / max(t, 1e-3))))
-        reps = max(1, int(args.cpu_seconds / max(t, 1e-3)))
-        reps = min(reps, 5)
+        reps = min(5, max(1, int(args.cpu_seconds / max(t, 1e-3))))
         times = []
         for _ in range(reps):
             t0 = time.perf_counter()
-            O.search(W["base_h"], g_h, pq_h[:m], W["pids"], hits_h[:m], po, n_threads=threads)
+            O.search(W["base_h"], g_h, pq_h[:m], W["pids"], ent_h[:m], po, n_threads=threads)
             times.append(time.perf_counter() - t0)
-        cpu_qps = m / float(np.median(times))
-        cpu = {"value": round(cpu_qps, 1), "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"first {m} of the {nq} queries, same PCA queries, entry leaves, graph "
-                         f"and parameters; search stage only; median of {reps} runs"}
+        cpu = {"value": round(m / float(np.median(times)), 1), "unit": "queries/s",
+               "cores": threads, "kind": "port",
+               "sample": f"first {m} of the {nq} queries with the same PCA queries, entry "
+                         f"leaves, graph and parameters as the headline run; search stage "
+                         f"only; median of {reps} runs"}
+
+    def summary(r):
+        if r is None:
+            return None
+        return {"qps": round(r["qps"], 1), "recall@10": round(r["recall"], 4), "beam": r["beam"],
+                "max_iter": r["max_iter"], "ms_per_step": round(r["ms"], 4),
+                "search_kernel_ms": round(r["search_ms"], 4), "mean_iterations": round(r["iters"], 2),
+                "predictor_ms": round(r["pred_ms"], 4) if r["pred_ms"] is not None else None}
 
     out = {
         "metric": METRIC,
-        "value": round(qps, 1),
+        "value": round(head["qps"], 1),
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(head["ms"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -256,17 +348,19 @@ def main():
             "workload": f"{W['n']}x{W['d']} integer-valued mixture (workload {args.workload}), "
                         f"{nq} queries, degree-{W['cfg']['degree']} kNN+reverse graph, "
                         f"{W['n_leaf']} kd-tree leaves, k={k}",
-            "entry": "kd-tree",
-            "beam": chosen[0], "max_iter": chosen[1], "width": 1,
-            "mode": "tuned" if args.mode else "ref",
-            "recall@10": round(final_recall, 4),
-            "search_kernel_ms": round(search_ms, 4),
-            "sweep": sweep,
+            "entry": "learned (kd-tree fallback)" if head is ln else "kd-tree",
+            "width": 1, "mode": "tuned" if args.mode else "ref",
+            "kd_tree_entry": summary(kd),
+            "learned_entry": summary(ln),
+            "learned_vs_kd_qps": round(ln["qps"] / kd["qps"], 3) if ln else None,
+            "predictor": pred_info,
+            "sweep_kd_tree": sweep_kd,
+            "sweep_learned": sweep_ln,
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
-            "kernel": "search_kernel", "algorithmic_bytes_per_launch": alg_bytes,
+            "kernel": "search_kernel", "algorithmic_bytes_per_launch": head["alg_bytes"],
         },
         "cpu_baseline": cpu,
     }

@@ -7,6 +7,8 @@
  *    the last failure on a context is rte_last_error(ctx).  Nothing calls
  *    exit() (the reference prints and exits, helper.h:4-34).
  *  - One context owns one HIP stream and the device buffers it allocated.
+ *    The stream is a blocking stream (ordered with the null stream); a
+ *    caller stream set with rte_set_stream() is ordered only by the caller.
  *    Calls on one context are not thread-safe; separate contexts are.
  *  - `where` says whether a pointer argument is host (RTE_HOST) or device
  *    (RTE_DEVICE) memory.  Host inputs are copied into context-owned device
@@ -149,6 +151,9 @@ int rte_mlp_train_step(rte_ctx* ctx, const float* X, const int32_t* y, int n, fl
  * softmax maximum is below min_conf. */
 int rte_mlp_predict(rte_ctx* ctx, const float* X, int64_t n, const int32_t* fallback,
                     float min_conf, int32_t* leaf_out, float* conf_out);
+/* Input normalisation x' = (x - mean) * inv_std per feature (host arrays of
+ * d_in), applied in FP32 before the bf16 rounding; identity until set. */
+int rte_mlp_set_input_norm(rte_ctx* ctx, const float* mean, const float* inv_std);
 /* Weights in/out as FP32 host arrays (W1 d_in x h1, b1, W2 h1 x h2, b2,
  * W3 h2 x n_out, b3), for checkpoints and the gradient check. */
 int rte_mlp_get(rte_ctx* ctx, float* W1, float* b1, float* W2, float* b2, float* W3,

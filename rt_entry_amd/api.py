@@ -241,3 +241,63 @@ class Context:
         self._chk(lib().rte_search_batched(self._h, pq, nq, qs, pl, pk, pb, C.byref(params),
                                            int(k_stride), pi, pd, w))
         return ids, dists
+
+    # ---- learned entry predictor (device tensors)
+    def mlp_create(self, d_in: int, h1: int, h2: int, n_out: int, seed: int = 1):
+        self._chk(lib().rte_mlp_create(self._h, int(d_in), int(h1), int(h2), int(n_out),
+                                       C.c_uint64(seed)))
+        self.mlp_dims = (int(d_in), int(h1), int(h2), int(n_out))
+
+    def mlp_set_input_norm(self, mean, inv_std):
+        m = np.ascontiguousarray(mean, np.float32)
+        v = np.ascontiguousarray(inv_std, np.float32)
+        self._chk(lib().rte_mlp_set_input_norm(self._h, m.ctypes.data_as(C.c_void_p),
+                                               v.ctypes.data_as(C.c_void_p)))
+
+    def mlp_get(self):
+        d = self.mlp_dims
+        W = [np.empty((d[i], d[i + 1]), np.float32) for i in range(3)]
+        b = [np.empty(d[i + 1], np.float32) for i in range(3)]
+        P = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._chk(lib().rte_mlp_get(self._h, P(W[0]), P(b[0]), P(W[1]), P(b[1]), P(W[2]),
+                                    P(b[2])))
+        return W, b
+
+    def mlp_set(self, W, b):
+        W = [np.ascontiguousarray(w, np.float32) for w in W]
+        b = [np.ascontiguousarray(x, np.float32) for x in b]
+        P = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._chk(lib().rte_mlp_set(self._h, P(W[0]), P(b[0]), P(W[1]), P(b[1]), P(W[2]),
+                                    P(b[2])))
+
+    def mlp_grad(self, X, y):
+        """Gradients of the mean cross-entropy on one device batch (host f32)."""
+        d = self.mlp_dims
+        gW = [np.empty((d[i], d[i + 1]), np.float32) for i in range(3)]
+        gb = [np.empty(d[i + 1], np.float32) for i in range(3)]
+        loss = C.c_float()
+        P = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._chk(lib().rte_mlp_grad(self._h, C.c_void_p(X.data_ptr()), C.c_void_p(y.data_ptr()),
+                                     int(X.shape[0]), P(gW[0]), P(gb[0]), P(gW[1]), P(gb[1]),
+                                     P(gW[2]), P(gb[2]), C.byref(loss)))
+        return gW, gb, loss.value
+
+    def mlp_train_step(self, X, y, lr: float, beta1: float = 0.9, want_loss: bool = True):
+        """One Adam step; returns the batch's mean loss (a host sync) or None."""
+        loss = C.c_float()
+        self._chk(lib().rte_mlp_train_step(self._h, C.c_void_p(X.data_ptr()),
+                                           C.c_void_p(y.data_ptr()), int(X.shape[0]),
+                                           float(lr), float(beta1),
+                                           C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def mlp_predict(self, X, fallback=None, min_conf: float = 0.5, out=None, conf=None):
+        import torch
+        n = X.shape[0]
+        if out is None:
+            out = torch.empty(n, dtype=torch.int32, device=X.device)
+        fb = C.c_void_p(fallback.data_ptr()) if fallback is not None else None
+        cp = C.c_void_p(conf.data_ptr()) if conf is not None else None
+        self._chk(lib().rte_mlp_predict(self._h, C.c_void_p(X.data_ptr()), int(n), fb,
+                                        float(min_conf), C.c_void_p(out.data_ptr()), cp))
+        return out

@@ -177,7 +177,10 @@ int rte_create(rte_ctx** out, int device) {
   if (device < 0 || device >= n) return RTE_EINVAL;
   rte_ctx* c = new rte_ctx();
   c->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+  // a blocking stream: ordered with the null stream, so work the caller
+  // queues there (torch's default stream) and the context's kernels see each
+  // other's results without explicit events
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->own) != hipSuccess) {
     delete c;
     return RTE_EHIP;
   }

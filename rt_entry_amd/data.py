@@ -134,6 +134,28 @@ def generate(cfg: int, scale: float = 1.0, n: int | None = None, nq: int | None 
     return out
 
 
+def extra_queries(cfg: int, n: int, what: int = 3):
+    """n more queries of workload cfg from stream `what` (3: training
+    queries), drawn around the same centres as the base."""
+    rc, rx = _rng(cfg, 0), _rng(cfg, what)
+    if cfg == 1:
+        cen = rc.uniform(0.0, 100.0, size=(64, 128))
+        return _mixture_f(None, rx, n, 128, 64, 0, 0, 8.0, centres=cen)[0]
+    if cfg in (2, 5):
+        ncen = 1000 if cfg == 2 else 10_000
+        cen = rc.uniform(0.0, 255.0, size=(ncen, 128))
+        return _mixture_int(None, rx, n, 128, ncen, 0.0, 255.0, 20.0, centres=cen)[0]
+    if cfg == 3:
+        c = rc.normal(size=(2000, 96))
+        cen = c / np.linalg.norm(c, axis=1, keepdims=True)
+        return _sphere(None, rx, n, 96, 2000, 0.15, centres=cen)[0]
+    if cfg == 4:
+        Wm, _ = np.linalg.qr(rc.normal(size=(960, 64)))
+        mu = rc.normal(0.0, 1.0, size=(500, 64))
+        return _lowrank(None, rx, n, 960, 500, 64, 0.25, 0.05, params=(Wm, mu))[0]
+    raise ValueError(cfg)
+
+
 def k_beam_assignment(cfg: int, nq: int, ks=(1, 10, 100), beams=(32, 64, 128, 256)):
     """Uniform per-query k and beam draws (seed +4); beam_eff = max(beam,
     pow2ceil(k))."""

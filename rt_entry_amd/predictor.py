@@ -1,0 +1,142 @@
+"""Learned entry-leaf predictor: training labels from the plain search, GPU
+training of the MLP (librte kernels), and inference with the kd-tree leaf as
+fallback.
+
+Labels.  For a training query every leaf is a candidate entry (or a given
+shortlist, e.g. the kd-tree leaf and the leaves of the exact 10 nearest
+neighbours).  Each candidate seeds the plain search under a ladder of
+iteration budgets; its score is the number of exact top-10 neighbours found,
+summed over the budgets -- the area under its recall-versus-hops curve -- so
+the label is the entry that converges in the fewest hops.  Ties keep the
+kd-tree leaf, then the lowest leaf id.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .api import Context, make_params
+
+
+def leaf_of_points(pids: np.ndarray, n: int) -> np.ndarray:
+    """belong[i] = leaf holding point i (from the ascending pid table)."""
+    belong = np.full(n, -1, np.int32)
+    leaf = np.repeat(np.arange(pids.shape[0], dtype=np.int32), pids.shape[1])
+    flat = pids.reshape(-1)
+    ok = flat < n
+    belong[flat[ok]] = leaf[ok]
+    return belong
+
+
+def candidate_leaves(kd_hits: np.ndarray, gt: np.ndarray, belong: np.ndarray, k: int = 10):
+    """(nq, 1 + k) candidate leaves, -1 padded: kd leaf then the distinct
+    leaves of the k exact neighbours."""
+    nq = kd_hits.shape[0]
+    nbr = belong[gt[:, :k]]
+    cand = np.full((nq, 1 + k), -1, np.int32)
+    cand[:, 0] = kd_hits
+    for q in range(nq):
+        seen = {int(kd_hits[q])}
+        c = 1
+        for x in nbr[q]:
+            x = int(x)
+            if x >= 0 and x not in seen:
+                seen.add(x)
+                cand[q, c] = x
+                c += 1
+    return cand
+
+
+def make_labels(ctx: Context, PQ, cand, gt, beam: int = 64, width: int = 1,
+                budgets=(4, 8, 16), mode: int = 0, kd=None, n_leaf: int | None = None,
+                rows_per_call: int = 1 << 22):
+    """Label per query (device search, device scoring).
+
+    cand: (nq, C) int32 shortlist, -1 padded, column 0 the kd-tree leaf; or
+    None to score all n_leaf leaves (kd = the kd-tree leaves for tie-breaks).
+    Returns (labels int32[nq], best score int32[nq])."""
+    import torch
+    dev = PQ.device
+    nq = PQ.shape[0]
+    gt10 = torch.as_tensor(np.ascontiguousarray(gt[:, :10]), device=dev).int()
+    params = [make_params(k=10, beam=beam, width=width, max_iter=b, mode=mode) for b in budgets]
+    if cand is None:
+        L = int(n_leaf)
+        kd_t = torch.as_tensor(np.ascontiguousarray(kd), device=dev).long()
+        labels = np.empty(nq, np.int32)
+        best = np.empty(nq, np.int32)
+        qc = max(1, rows_per_call // L)
+        leaf_ids = torch.arange(L, device=dev, dtype=torch.int32)
+        for q0 in range(0, nq, qc):
+            qn = min(qc, nq - q0)
+            qi = torch.arange(q0, q0 + qn, device=dev).repeat_interleave(L)
+            X = PQ[qi].contiguous()
+            leaf = leaf_ids.repeat(qn).contiguous()
+            g = gt10[qi]
+            hits = torch.zeros(qn * L, dtype=torch.int64, device=dev)
+            for p in params:
+                ids = ctx.search(X, leaf, p, with_dists=False)
+                hits += (ids[:, :, None] == g[:, None, :]).any(-1).sum(1)
+            hits = hits.view(qn, L)
+            # unique key: hits, then the kd-tree leaf, then the lowest id
+            key = hits * (2 * L) + (L - 1 - leaf_ids.long())[None, :]
+            key[torch.arange(qn, device=dev), kd_t[q0:q0 + qn]] += L
+            arg = key.argmax(1)
+            labels[q0:q0 + qn] = arg.int().cpu().numpy()
+            best[q0:q0 + qn] = hits.gather(1, arg[:, None])[:, 0].int().cpu().numpy()
+            del X, g, hits, key
+        return labels, best
+    C = cand.shape[1]
+    qi_np, ci_np = np.nonzero(cand >= 0)
+    leaves = torch.from_numpy(cand[qi_np, ci_np].astype(np.int32)).to(dev)
+    qidx = torch.from_numpy(qi_np.astype(np.int64)).to(dev)
+    hits = torch.zeros(qi_np.size, dtype=torch.int64, device=dev)
+    for p in params:
+        for s in range(0, qi_np.size, rows_per_call):
+            sl = slice(s, min(qi_np.size, s + rows_per_call))
+            ids = ctx.search(PQ[qidx[sl]].contiguous(), leaves[sl].contiguous(), p,
+                             with_dists=False)
+            hits[sl] += (ids[:, :, None] == gt10[qidx[sl]][:, None, :]).any(-1).sum(1)
+    h = hits.cpu().numpy()
+    order = np.lexsort((ci_np, -h, qi_np))
+    first = np.ones(order.size, bool)
+    first[1:] = qi_np[order][1:] != qi_np[order][:-1]
+    top = order[first]
+    labels = np.empty(nq, np.int32)
+    best = np.empty(nq, np.int32)
+    labels[qi_np[top]] = cand[qi_np[top], ci_np[top]]
+    best[qi_np[top]] = h[top]
+    return labels, best
+
+
+def train(ctx: Context, X, y, n_out: int, hidden=(256, 256), epochs: int = 20,
+          batch: int = 1024, lr: float = 2e-3, seed: int = 1, log=None):
+    """Trains the MLP on device tensors X (n x d) and labels y (n).  The
+    input is standardised per feature with the training set's mean and
+    standard deviation (kept in the network)."""
+    import torch
+    n, d = X.shape
+    ctx.mlp_create(d, hidden[0], hidden[1], n_out, seed=seed)
+    Xh = X.double()
+    mean = Xh.mean(0)
+    std = Xh.std(0).clamp_min(1e-6)
+    ctx.mlp_set_input_norm(mean.float().cpu().numpy(), (1.0 / std).float().cpu().numpy())
+    g = torch.Generator(device="cpu")
+    g.manual_seed(seed)
+    losses = []
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g).to(X.device)
+        Xs, ys = X[perm].contiguous(), y[perm].contiguous()
+        last = None
+        for s in range(0, n - batch + 1, batch):
+            want = log is not None or (ep == epochs - 1 and s + 2 * batch > n)
+            l = ctx.mlp_train_step(Xs[s:s + batch], ys[s:s + batch], lr=lr, want_loss=want)
+            if l is not None:
+                last = l
+        losses.append(last)
+        if log:
+            log(f"epoch {ep} loss {last:.4f}")
+    return losses
+
+
+def predict(ctx: Context, X, fallback, min_conf: float = 0.5, conf=None):
+    return ctx.mlp_predict(X, fallback=fallback, min_conf=min_conf, conf=conf)

@@ -1,0 +1,69 @@
+"""GPU entry predictor: gradients of the HIP forward/backward kernels against
+the FP64 oracle (same bf16 roundings), training progress, and prediction with
+the kd-tree fallback."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+
+
+@pytest.mark.parametrize("shape", [(40, 48, 56, 100), (128, 256, 256, 512)])
+def test_mlp_gradients_match_fp64_oracle(ctx, shape):
+    import torch
+    from oracle import mlp as M
+    rng = np.random.default_rng(11)
+    n = 70
+    ctx.mlp_create(*shape, seed=5)
+    W, b = ctx.mlp_get()
+    b = [rng.normal(0, 0.1, size=x.shape).astype(np.float32) for x in b]
+    ctx.mlp_set(W, b)
+    X = rng.normal(size=(n, shape[0])).astype(np.float32)
+    y = rng.integers(0, shape[3], size=n).astype(np.int32)
+    Xd = torch.from_numpy(X).cuda()
+    yd = torch.from_numpy(y).cuda()
+    mean = rng.normal(size=shape[0]).astype(np.float32)
+    inv = rng.uniform(0.5, 2.0, size=shape[0]).astype(np.float32)
+    ctx.mlp_set_input_norm(mean, inv)
+    gW, gb, loss = ctx.mlp_grad(Xd, yd)
+    oW, ob, oloss = M.grads(W, b, X, y, norm=(mean, inv))
+    print(f"[mlp] shape={shape} loss gpu={loss} oracle={oloss}", flush=True)
+    assert abs(loss - oloss) < 1e-4 * max(1.0, abs(oloss))
+    for l in range(3):
+        # FP32 accumulation over K terms against exact FP64 sums
+        assert _rel(gW[l], oW[l]) < 2e-4, (l, _rel(gW[l], oW[l]))
+        assert _rel(gb[l], ob[l]) < 2e-4, (l, _rel(gb[l], ob[l]))
+
+
+def test_mlp_training_reduces_loss_and_predicts(ctx):
+    import torch
+    from oracle import mlp as M
+    rng = np.random.default_rng(7)
+    d, C, n = 32, 64, 4096
+    centres = rng.normal(size=(C, d)).astype(np.float32) * 3
+    y = rng.integers(0, C, size=n).astype(np.int32)
+    X = (centres[y] + rng.normal(size=(n, d))).astype(np.float32)
+    ctx.mlp_create(d, 64, 64, C, seed=1)
+    Xd = torch.from_numpy(X).cuda()
+    yd = torch.from_numpy(y).cuda()
+    losses = []
+    for ep in range(30):
+        for s in range(0, n, 512):
+            losses.append(ctx.mlp_train_step(Xd[s:s + 512], yd[s:s + 512], lr=3e-3))
+    print(f"[mlp] loss {losses[0]:.3f} -> {losses[-1]:.3f}", flush=True)
+    assert losses[-1] < 0.3 * losses[0]
+    fb = torch.full((n,), 7, dtype=torch.int32, device="cuda")
+    conf = torch.empty(n, dtype=torch.float32, device="cuda")
+    leaf = ctx.mlp_predict(Xd, fallback=fb, min_conf=0.5, conf=conf).cpu().numpy()
+    assert (leaf == y).mean() > 0.9
+    W, b = ctx.mlp_get()
+    ol, op = M.predict(W, b, X, fallback=np.full(n, 7, np.int32), min_conf=0.5)
+    agree = (ol == leaf).mean()
+    assert agree > 0.995, agree
+    assert np.abs(conf.cpu().numpy() - op).max() < 1e-3
+    # a confidence threshold above 1 sends every query to the fallback
+    leaf2 = ctx.mlp_predict(Xd, fallback=fb, min_conf=1.5).cpu().numpy()
+    assert (leaf2 == 7).all()
