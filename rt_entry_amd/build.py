@@ -22,6 +22,8 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librte.so")
+DRIVER = os.path.join(HERE, "rte_search")
+HOST = os.path.join(HERE, "host")
 
 HIP_SOURCES = ["api.hip", "search.hip", "search_nr1.hip", "search_nr2.hip", "search_nr4.hip",
                "search_nr8.hip", "entry.hip", "project.hip", "mlp.hip"]
@@ -71,6 +73,14 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lgomp"])
     os.replace(tmp, LIB)
+    # host C++ API (Graph mirror) and the reference-style driver over the ABI
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    tmp = DRIVER + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-D__HIP_PLATFORM_AMD__", f"-I{INCLUDE}",
+          f"-I{rocm}/include", os.path.join(HOST, "graph.cpp"), os.path.join(HOST, "main.cpp"),
+          "-o", tmp, f"-L{HERE}", "-lrte", f"-L{rocm}/lib", "-lamdhip64",
+          "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{rocm}/lib"])
+    os.replace(tmp, DRIVER)
     return LIB
 
 

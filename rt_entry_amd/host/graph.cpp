@@ -1,0 +1,448 @@
+// Host C++ API of the query path over the C ABI (see graph.h).
+#include "graph.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <chrono>
+#include <fstream>
+#include <numeric>
+#include <sstream>
+
+namespace rte_host {
+
+namespace {
+
+bool ends_with(const std::string& s, const std::string& t) {
+  return s.size() >= t.size() && s.compare(s.size() - t.size(), t.size(), t) == 0;
+}
+
+bool exists(const std::string& p) {
+  struct stat st;
+  return stat(p.c_str(), &st) == 0;
+}
+
+int64_t file_size(std::ifstream& f) {
+  f.seekg(0, std::ios::end);
+  const int64_t n = f.tellg();
+  f.seekg(0, std::ios::beg);
+  return n;
+}
+
+std::ifstream open_in(const std::string& path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f.is_open()) throw Error("cannot open " + path);
+  return f;
+}
+
+void hip_ok(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// device buffer owned by the host API
+struct Dev {
+  void* p = nullptr;
+  ~Dev() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+// file_read.cu:121-145 (fvecs), :147-161 (fbin), :184-211 (bvecs, at most
+// 100M rows), :82-101 (hh: int32 d then floats to EOF)
+void read_data(const std::string& path, int& n, int& d, std::vector<float>& out) {
+  std::ifstream f = open_in(path);
+  const int64_t size = file_size(f);
+  if (ends_with(path, "fvecs")) {
+    int32_t dd = 0;
+    f.read((char*)&dd, 4);
+    if (dd <= 0) throw Error("bad fvecs header in " + path);
+    d = dd;
+    n = (int)(size / (4LL * (d + 1)));
+    out.resize((size_t)n * d);
+    f.seekg(0);
+    for (int i = 0; i < n; ++i) {
+      f.seekg(4, std::ios::cur);
+      f.read((char*)&out[(size_t)i * d], 4LL * d);
+    }
+  } else if (ends_with(path, "fbin")) {
+    int32_t hdr[2];
+    f.read((char*)hdr, 8);
+    n = hdr[0];
+    d = hdr[1];
+    if (n <= 0 || d <= 0 || 8 + 4LL * n * d > size) throw Error("bad fbin header in " + path);
+    out.resize((size_t)n * d);
+    f.read((char*)out.data(), 4LL * n * d);
+  } else if (ends_with(path, "bvecs")) {
+    int32_t dd = 0;
+    f.read((char*)&dd, 4);
+    if (dd <= 0) throw Error("bad bvecs header in " + path);
+    d = dd;
+    n = (int)std::min<int64_t>(size / (4 + d), 100000000);
+    out.resize((size_t)n * d);
+    std::vector<uint8_t> row(d);
+    f.seekg(0);
+    for (int i = 0; i < n; ++i) {
+      f.seekg(4, std::ios::cur);
+      f.read((char*)row.data(), d);
+      for (int j = 0; j < d; ++j) out[(size_t)i * d + j] = row[j];
+    }
+  } else if (ends_with(path, "hh")) {
+    int32_t dd = 0;
+    f.read((char*)&dd, 4);
+    d = dd;
+    if (d <= 0) throw Error("bad hh header in " + path);
+    n = (int)((size - 4) / (4LL * d));
+    out.resize((size_t)n * d);
+    f.read((char*)out.data(), 4LL * n * d);
+  } else {
+    throw Error("unknown data file type: " + path);
+  }
+  if (!f) throw Error("short read in " + path);
+}
+
+// file_read.cu:213-235
+void read_ivecs(const std::string& path, int& n, int& d, std::vector<int32_t>& out) {
+  std::ifstream f = open_in(path);
+  const int64_t size = file_size(f);
+  int32_t dd = 0;
+  f.read((char*)&dd, 4);
+  if (dd <= 0) throw Error("bad ivecs header in " + path);
+  d = dd;
+  n = (int)(size / (4LL * (d + 1)));
+  out.resize((size_t)n * d);
+  f.seekg(0);
+  for (int i = 0; i < n; ++i) {
+    f.seekg(4, std::ios::cur);
+    f.read((char*)&out[(size_t)i * d], 4LL * d);
+  }
+  if (!f) throw Error("short read in " + path);
+}
+
+// file_read.cu:237-255: raw int32 rows, degree = file size / (4 n)
+void read_graph(const std::string& path, int n, int& degree, std::vector<int32_t>& out) {
+  std::ifstream f = open_in(path);
+  const int64_t size = file_size(f);
+  degree = (int)(size / (4LL * n));
+  if (degree <= 0) throw Error("graph file too short: " + path);
+  out.resize((size_t)n * degree);
+  f.read((char*)out.data(), 4LL * n * degree);
+  if (!f) throw Error("short read in " + path);
+}
+
+// pca.cpp:70-89, 128-139: uint32 n, uint32 d, then n*d floats
+void write_fbin(const std::string& path, uint32_t n, uint32_t d, const float* data) {
+  std::ofstream f(path, std::ios::binary);
+  if (!f.is_open()) throw Error("cannot write " + path);
+  f.write((const char*)&n, 4);
+  f.write((const char*)&d, 4);
+  f.write((const char*)data, 4LL * n * d);
+}
+
+// cyclic Jacobi eigen-decomposition of a symmetric d x d matrix (FP64)
+static void jacobi_eigen(std::vector<double>& A, int d, std::vector<double>& V,
+                         std::vector<double>& w) {
+  V.assign((size_t)d * d, 0.0);
+  for (int i = 0; i < d; ++i) V[(size_t)i * d + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < d; ++j)
+        (i == j ? diag : off) += A[(size_t)i * d + j] * A[(size_t)i * d + j];
+    if (off <= 1e-30 * diag || off == 0.0) break;
+    for (int p = 0; p < d - 1; ++p)
+      for (int q = p + 1; q < d; ++q) {
+        const double apq = A[(size_t)p * d + q];
+        if (fabs(apq) < 1e-300) continue;
+        const double app = A[(size_t)p * d + p], aqq = A[(size_t)q * d + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < d; ++k) {  // rows p, q
+          const double akp = A[(size_t)k * d + p], akq = A[(size_t)k * d + q];
+          A[(size_t)k * d + p] = c * akp - s * akq;
+          A[(size_t)k * d + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < d; ++k) {  // columns p, q
+          const double apk = A[(size_t)p * d + k], aqk = A[(size_t)q * d + k];
+          A[(size_t)p * d + k] = c * apk - s * aqk;
+          A[(size_t)q * d + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < d; ++k) {
+          const double vkp = V[(size_t)k * d + p], vkq = V[(size_t)k * d + q];
+          V[(size_t)k * d + p] = c * vkp - s * vkq;
+          V[(size_t)k * d + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  w.resize(d);
+  for (int i = 0; i < d; ++i) w[i] = A[(size_t)i * d + i];
+}
+
+void pca_fit(const float* X, int64_t n, int d, std::vector<float>& R, std::vector<float>& mu,
+             std::vector<double>* eigenvalues) {
+  std::vector<double> mean(d, 0.0), C((size_t)d * d, 0.0);
+#pragma omp parallel
+  {
+    std::vector<double> local(d, 0.0);
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < n; ++i)
+      for (int j = 0; j < d; ++j) local[j] += X[i * d + j];
+#pragma omp critical
+    for (int j = 0; j < d; ++j) mean[j] += local[j];
+  }
+  for (int j = 0; j < d; ++j) mean[j] /= (double)n;
+#pragma omp parallel
+  {
+    std::vector<double> local((size_t)d * d, 0.0), x(d);
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      for (int j = 0; j < d; ++j) x[j] = (double)X[i * d + j] - mean[j];
+      for (int a = 0; a < d; ++a)
+        for (int b = a; b < d; ++b) local[(size_t)a * d + b] += x[a] * x[b];
+    }
+#pragma omp critical
+    for (size_t k = 0; k < local.size(); ++k) C[k] += local[k];
+  }
+  for (int a = 0; a < d; ++a)
+    for (int b = a; b < d; ++b) {
+      C[(size_t)a * d + b] /= (double)std::max<int64_t>(1, n - 1);  // pca.cpp:8-9
+      C[(size_t)b * d + a] = C[(size_t)a * d + b];
+    }
+  std::vector<double> V, w;
+  jacobi_eigen(C, d, V, w);
+  std::vector<int> order(d);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
+  R.assign((size_t)d * d, 0.f);
+  for (int j = 0; j < d; ++j) {
+    const int src = order[j];
+    // deterministic sign: largest-magnitude component positive
+    int arg = 0;
+    for (int i = 1; i < d; ++i)
+      if (fabs(V[(size_t)i * d + src]) > fabs(V[(size_t)arg * d + src])) arg = i;
+    const double sg = V[(size_t)arg * d + src] < 0 ? -1.0 : 1.0;
+    for (int i = 0; i < d; ++i) R[(size_t)i * d + j] = (float)(sg * V[(size_t)i * d + src]);
+  }
+  mu.resize(d);
+  for (int j = 0; j < d; ++j) mu[j] = (float)mean[j];
+  if (eigenvalues) {
+    eigenvalues->resize(d);
+    for (int j = 0; j < d; ++j) (*eigenvalues)[j] = w[order[j]];
+  }
+}
+
+Graph::Graph(int /*n_subspaces*/, int /*buffer_size*/, int n_candidates, int /*max_hits*/,
+             double expand_ratio, double point_ratio, std::string data_name,
+             std::string& data_path, std::string& query_path, std::string& gt_path,
+             std::string& graph_path, int ALGO, int search_width, int topk, int max_iter,
+             int device, int mode)
+    : n_candidates_(n_candidates),
+      topk_(topk),
+      max_iter_(max_iter),
+      search_width_(search_width),
+      algo_(ALGO),
+      mode_(mode),
+      expand_ratio_((float)expand_ratio),
+      point_ratio_((float)point_ratio),
+      data_name_(std::move(data_name)),
+      data_path_(data_path),
+      query_path_(query_path),
+      gt_path_(gt_path),
+      graph_path_(graph_path) {
+  // graph.cu:53-54: beam rounded up to a power of two; topk 0 means beam
+  int nc = 1;
+  while (nc < n_candidates_) nc <<= 1;
+  n_candidates_ = nc;
+  if (topk_ == 0) topk_ = n_candidates_;
+  check(rte_create(&ctx_, device), "rte_create");
+}
+
+Graph::~Graph() { CleanUp(); }
+
+void Graph::check(int rc, const char* what) {
+  if (rc != RTE_OK) {
+    std::ostringstream os;
+    os << what << " failed (" << rc << "): " << (ctx_ ? rte_last_error(ctx_) : "");
+    throw Error(os.str());
+  }
+}
+
+void Graph::log_line(const std::string& s) {
+  printf("%s", s.c_str());
+  std::ofstream f(out_file_, std::ios::app);
+  f << s << std::flush;
+}
+
+void Graph::Input() {  // graph.cu:73-112
+  read_data(data_path_, np_, dim_, points_);
+  int qd = 0;
+  read_data(query_path_, nq_, qd, queries_);
+  if (qd != dim_) throw Error("query and base dimensions differ");
+  read_ivecs(gt_path_, nq_ == 0 ? nq_ : nq_, gt_k_, gt_);
+  if ((int64_t)gt_.size() / std::max(1, gt_k_) < nq_) throw Error("ground truth has too few rows");
+  if (search_dim_ <= 0 || search_dim_ > dim_) search_dim_ = dim_;
+}
+
+void Graph::RB_Graph() {  // graph.cu:114-139
+  if (!exists(graph_path_)) throw Error("graph file missing (graph building is not part of the path): " + graph_path_);
+  read_graph(graph_path_, np_, degree_, graph_);
+  check(rte_set_graph(ctx_, graph_.data(), np_, degree_, RTE_HOST), "rte_set_graph");
+}
+
+void Graph::Projection() {  // graph.cu:141-252, pca.cpp
+  if (algo_ == 0) {  // raw space (graph.cu:306-310)
+    check(rte_set_base(ctx_, points_.data(), np_, dim_, dim_, RTE_HOST), "rte_set_base");
+    return;
+  }
+  const std::string rot = data_name_ + "/O.fbin", mean = data_name_ + "/mean.fbin",
+                    pb = data_name_ + "/pca_base.fbin";
+  if (exists(rot) && exists(mean)) {
+    int n1, d1;
+    read_data(rot, n1, d1, R_);
+    if (n1 != dim_ || d1 != dim_) throw Error("rotation shape mismatch in " + rot);
+    read_data(mean, n1, d1, mu_);
+    if (n1 != 1 || d1 != dim_) throw Error("mean shape mismatch in " + mean);
+  } else {
+    pca_fit(points_.data(), np_, dim_, R_, mu_);
+    mkdir(data_name_.c_str(), 0755);
+    write_fbin(rot, dim_, dim_, R_.data());
+    write_fbin(mean, 1, dim_, mu_.data());
+  }
+  if (exists(pb)) {
+    int n1, d1;
+    read_data(pb, n1, d1, pca_points_);
+    if (n1 != np_ || d1 != dim_) throw Error("pca_base shape mismatch in " + pb);
+  } else {
+    pca_points_.resize((size_t)np_ * dim_);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < np_; ++i)
+      for (int c = 0; c < dim_; ++c) {
+        double s = 0.0;
+        for (int j = 0; j < dim_; ++j)
+          s += ((double)points_[i * dim_ + j] - mu_[j]) * (double)R_[(size_t)j * dim_ + c];
+        pca_points_[i * dim_ + c] = (float)s;
+      }
+    write_fbin(pb, np_, dim_, pca_points_.data());
+  }
+  // search space: the first search_dim_ PCA columns, row stride dim_
+  check(rte_set_base(ctx_, pca_points_.data(), np_, search_dim_, dim_, RTE_HOST), "rte_set_base");
+  check(rte_set_pca(ctx_, R_.data(), mu_.data(), dim_, dim_, search_dim_, RTE_HOST), "rte_set_pca");
+}
+
+void Graph::Init_entry() {  // entry.cu:8-98
+  if (algo_ != 1) return;
+  const int cap = (int)((float)np_ * point_ratio_);  // entry.cu:22
+  if (cap <= 0) throw Error("point_ratio gives an empty leaf capacity");
+  const std::string cache = data_name_ + "/" + std::to_string(cap) + "_0.aabbs";
+  std::vector<float> boxes;
+  std::vector<int32_t> pids;
+  int n_leaf = 0;
+  if (exists(cache)) {  // entry.cu:183-190
+    std::ifstream f = open_in(cache);
+    int32_t hdr[2];
+    f.read((char*)hdr, 8);
+    n_leaf = hdr[0];
+    if (hdr[1] != cap || n_leaf <= 0) throw Error("aabb cache shape mismatch in " + cache);
+    boxes.resize((size_t)n_leaf * 6);
+    pids.resize((size_t)n_leaf * cap);
+    f.read((char*)boxes.data(), 24LL * n_leaf);
+    f.read((char*)pids.data(), 4LL * n_leaf * cap);
+    if (!f) throw Error("short read in " + cache);
+  } else {
+    std::vector<float> pts3((size_t)np_ * 3);
+    for (int64_t i = 0; i < np_; ++i)
+      for (int a = 0; a < 3; ++a) pts3[i * 3 + a] = pca_points_[i * dim_ + a];
+    const int max_leaves = 4 * (np_ / cap) + 16;
+    boxes.resize((size_t)max_leaves * 6);
+    pids.resize((size_t)max_leaves * cap);
+    check(rte_build_kdtree(pts3.data(), np_, cap, max_leaves, boxes.data(), pids.data(), &n_leaf),
+          "rte_build_kdtree");
+    boxes.resize((size_t)n_leaf * 6);
+    pids.resize((size_t)n_leaf * cap);
+    mkdir(data_name_.c_str(), 0755);
+    std::ofstream f(cache, std::ios::binary);  // entry.cu:171-180
+    const int32_t hdr[2] = {n_leaf, cap};
+    f.write((const char*)hdr, 8);
+    f.write((const char*)boxes.data(), 24LL * n_leaf);
+    f.write((const char*)pids.data(), 4LL * n_leaf * cap);
+  }
+  log_line("n_aabbs: " + std::to_string(n_leaf) + "\n");  // entry.cu:55-58
+  rte_expand_aabbs(boxes.data(), n_leaf, expand_ratio_);
+  check(rte_set_leaves(ctx_, boxes.data(), pids.data(), n_leaf, cap, RTE_HOST), "rte_set_leaves");
+}
+
+void Graph::Search() {  // graph.cu:254-293
+  const int qdim = algo_ == 0 ? dim_ : search_dim_;
+  Dev dq, dpq, dleaf, dids, ddist;
+  hip_ok(hipMalloc(&dq.p, (size_t)nq_ * dim_ * 4), "hipMalloc queries");
+  hip_ok(hipMemcpy(dq.p, queries_.data(), (size_t)nq_ * dim_ * 4, hipMemcpyHostToDevice),
+         "copy queries");
+  hip_ok(hipMalloc(&dpq.p, (size_t)nq_ * std::max(qdim, 3) * 4), "hipMalloc pq");
+  hip_ok(hipMalloc(&dleaf.p, (size_t)nq_ * 4), "hipMalloc leaf");
+  hip_ok(hipMalloc(&dids.p, (size_t)nq_ * topk_ * 4), "hipMalloc ids");
+  hip_ok(hipMalloc(&ddist.p, (size_t)nq_ * topk_ * 4), "hipMalloc dists");
+  rte_params p{};
+  p.k = topk_;
+  p.beam = n_candidates_;
+  p.width = search_width_;
+  p.max_iter = max_iter_;
+  p.mode = mode_;
+  p.algo = algo_;
+  check(rte_synchronize(ctx_), "sync");
+  const auto t0 = std::chrono::high_resolution_clock::now();
+  const float* sq = (const float*)dq.p;
+  if (algo_ != 0) {
+    check(rte_project(ctx_, (const float*)dq.p, nq_, (float*)dpq.p, RTE_DEVICE), "rte_project");
+    sq = (const float*)dpq.p;
+  }
+  if (algo_ == 1)
+    check(rte_select_entry(ctx_, sq, nq_, qdim, (int32_t*)dleaf.p, RTE_DEVICE),
+          "rte_select_entry");
+  check(rte_search(ctx_, sq, nq_, qdim, (const int32_t*)dleaf.p, &p, (int32_t*)dids.p,
+                   (float*)ddist.p, nullptr, RTE_DEVICE),
+        "rte_search");
+  check(rte_synchronize(ctx_), "sync");
+  const auto t1 = std::chrono::high_resolution_clock::now();
+  search_ms_ = std::chrono::duration<double>(t1 - t0).count() * 1000.0;
+  std::ostringstream os;
+  os << "time search: " << search_ms_ << " ms\n";  // Timing.h:41-46
+  log_line(os.str());
+  results_.resize((size_t)nq_ * topk_);
+  dists_.resize((size_t)nq_ * topk_);
+  hits_.resize(nq_);
+  hip_ok(hipMemcpy(results_.data(), dids.p, results_.size() * 4, hipMemcpyDeviceToHost), "ids");
+  hip_ok(hipMemcpy(dists_.data(), ddist.p, dists_.size() * 4, hipMemcpyDeviceToHost), "dists");
+  if (algo_ == 1)
+    hip_ok(hipMemcpy(hits_.data(), dleaf.p, hits_.size() * 4, hipMemcpyDeviceToHost), "leaves");
+  // graph_helper.cu:64-121: true top-10 found among the topk results
+  int64_t found = 0;
+  const int g10 = std::min(10, gt_k_);
+  for (int q = 0; q < nq_; ++q)
+    for (int i = 0; i < g10; ++i) {
+      const int32_t g = gt_[(size_t)q * gt_k_ + i];
+      for (int j = 0; j < topk_; ++j)
+        if (results_[(size_t)q * topk_ + j] == g) {
+          ++found;
+          break;
+        }
+    }
+  recall10_ = (double)found / nq_ / 10.0;
+  std::ostringstream rs;
+  rs << "recall@10 = " << recall10_ << " ms\n";  // graph_helper.cu:118, suffix as written
+  log_line(rs.str());
+}
+
+void Graph::CleanUp() {
+  if (ctx_) {
+    rte_destroy(ctx_);
+    ctx_ = nullptr;
+  }
+}
+
+}  // namespace rte_host
