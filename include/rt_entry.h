@@ -54,6 +54,9 @@ typedef struct rte_params {
   int algo;         /* 1: leaf entry table; 0 or 2: per-lane LCG entries */
   int bloom_words;  /* 0: default (REF 512, TUNED auto_tune_bloom sizing) */
   int bloom_hashes; /* 0, or the mode's count: REF 3, TUNED 7 */
+  int wg_waves;     /* queries (wave64s) per workgroup, 1..4; 0: 4 */
+  int grid;         /* workgroups launched; 0: one per wg_waves queries
+                       (rte_search) or the resident maximum (batched) */
 } rte_params;
 
 /* ---- context ---------------------------------------------------------- */
@@ -132,6 +135,19 @@ int rte_search_batched(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stri
                        const int32_t* leaf, const int32_t* k_per_q,
                        const int32_t* beam_per_q, const rte_params* p, int k_stride,
                        int32_t* ids_out, float* dists_out, int where);
+
+/* Autotuner for rte_search_batched: times the batch (device buffers) under
+ * every launch shape (wg_waves 1/2/4 x persistent grid x1/x2) and every
+ * filter size in bloom_words[0..n_bloom) (0 = the mode's default), and
+ * writes the fastest configuration to *best (a copy of *p with wg_waves,
+ * grid and bloom_words set) and its time to *best_ms.  Launch shape never
+ * changes results; the filter size can, so callers pass only sizes they
+ * accept.  ids_out/dists_out receive the last run's results. */
+int rte_autotune_batched(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stride,
+                         const int32_t* leaf, const int32_t* k_per_q,
+                         const int32_t* beam_per_q, const rte_params* p, int k_stride,
+                         int32_t* ids_out, float* dists_out, const int* bloom_words,
+                         int n_bloom, rte_params* best, float* best_ms);
 
 /* Whole query path on device buffers (graph.cu:254-293 Graph::Search):
  * project, select entries (or take the predictor's leaves), search. */

@@ -41,9 +41,9 @@ def _ptr(x, dtype):
 
 
 def make_params(k=10, beam=64, width=1, max_iter=100, mode=MODE_REF, algo=1, bloom_words=0,
-                bloom_hashes=0) -> Params:
+                bloom_hashes=0, wg_waves=0, grid=0) -> Params:
     return Params(int(k), int(beam), int(width), int(max_iter), int(mode), int(algo),
-                  int(bloom_words), int(bloom_hashes))
+                  int(bloom_words), int(bloom_hashes), int(wg_waves), int(grid))
 
 
 def hash_parameter(nc: int):
@@ -301,3 +301,23 @@ class Context:
         self._chk(lib().rte_mlp_predict(self._h, C.c_void_p(X.data_ptr()), int(n), fb,
                                         float(min_conf), C.c_void_p(out.data_ptr()), cp))
         return out
+
+    def autotune_batched(self, PQ, leaf, k_per_q, beam_per_q, params: Params, k_stride=None,
+                         bloom_words=(0,)):
+        """Fastest (wg_waves, grid, bloom_words) for this batch (device
+        tensors).  Returns (best Params, best ms)."""
+        import torch
+        nq, qs = PQ.shape
+        if k_stride is None:
+            k_stride = int(k_per_q.max())
+        ids = torch.empty((nq, k_stride), dtype=torch.int32, device=PQ.device)
+        dists = torch.empty((nq, k_stride), dtype=torch.float32, device=PQ.device)
+        bw = (C.c_int * len(bloom_words))(*[int(x) for x in bloom_words])
+        best = Params()
+        ms = C.c_float()
+        P = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        self._chk(lib().rte_autotune_batched(self._h, P(PQ), nq, qs, P(leaf), P(k_per_q),
+                                             P(beam_per_q), C.byref(params), int(k_stride),
+                                             P(ids), P(dists), bw, len(bloom_words),
+                                             C.byref(best), C.byref(ms)))
+        return best, ms.value

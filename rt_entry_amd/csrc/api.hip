@@ -400,6 +400,49 @@ int rte_select_entry(rte_ctx* c, const float* PQ, int64_t nq, int stride, int32_
   return finish_out(c, leaf_out, dl, (size_t)nq, where);
 }
 
+__global__ void reset_counter_kernel(int64_t* counter) {
+  // one thread; the address depends on threadIdx so it is a vector store
+  counter[threadIdx.x] = 0;
+}
+
+// Counter reset + persistent search kernel, captured once per launch shape as
+// a hipGraph and replayed (the reset is a kernel node, not a memset node).
+static int launch_batched_graph(rte_ctx* c, const rte::SearchArgs& a, int beam_max, int grid,
+                                int wpb) {
+  std::vector<unsigned char> key(sizeof(a) + 4 * sizeof(int) + sizeof(void*));
+  unsigned char* k = key.data();
+  memcpy(k, &a, sizeof(a));
+  const int extra[4] = {beam_max, grid, wpb, c->device};
+  memcpy(k + sizeof(a), extra, sizeof(extra));
+  void* sp = (void*)c->stream;
+  memcpy(k + sizeof(a) + sizeof(extra), &sp, sizeof(sp));
+  if (!c->gexec || key != c->gkey) {
+    if (c->gexec) {
+      (void)hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed), "begin capture");
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, a.work_counter);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = rte::launch_search(a, beam_max, grid, wpb, c->stream);
+    const hipError_t e3 = hipStreamEndCapture(c->stream, &g);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return hip_fail(c, e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3, "capture batched");
+    }
+    const hipError_t e4 = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e4 != hipSuccess) {
+      c->gexec = nullptr;
+      return hip_fail(c, e4, "instantiate batched graph");
+    }
+    c->gkey = key;
+  }
+  HIPCHK(c, hipGraphLaunch(c->gexec, c->stream), "graph launch");
+  return RTE_OK;
+}
+
 static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride,
                          const int32_t* leaf, const int32_t* k_per_q, const int32_t* beam_per_q,
                          const rte_params* p, int k_stride, int32_t* ids_out, float* dists_out,
@@ -505,26 +548,51 @@ static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
     if (int r = stage_out(c, c->s_aux2, dists_out, (size_t)nq * a.k_stride, where, &dd)) return r;
   if (stats_out)
     if (int r = stage_out(c, c->s_aux3, stats_out, (size_t)nq * 3, where, &dst)) return r;
-  DevBuf kq, bq;
-  if (uneven) {
-    HIPCHK(c, kq.reserve(nq * sizeof(int32_t)), "hipMalloc(k_per_q)");
-    HIPCHK(c, bq.reserve(nq * sizeof(int32_t)), "hipMalloc(beam_per_q)");
-    HIPCHK(c, hipMemcpyAsync(kq.p, hk.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice, c->stream),
-           "copy k_per_q");
-    HIPCHK(c, hipMemcpyAsync(bq.p, hb.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice, c->stream),
-           "copy beam_per_q");
-    a.k_per_q = kq.as<int32_t>();
-    a.beam_per_q = bq.as<int32_t>();
-  }
+  const int wpb = p->wg_waves > 0 ? p->wg_waves : 4;
+  if (wpb > 4) return fail(c, RTE_EINVAL, "wg_waves %d outside 1..4", p->wg_waves);
+  if (p->grid < 0) return fail(c, RTE_EINVAL, "grid < 0");
   a.queries = dq;
   a.hits = dleaf;
   a.ids_out = dids;
   a.dists_out = dd;
   a.stats_out = dst;
-  const int64_t blocks = (nq + 3) / 4;
-  const int grid = (int)std::min<int64_t>(blocks, 1 << 20);
-  HIPCHK(c, rte::launch_search(a, beam_max, grid, c->stream), "search launch");
-  if (uneven) HIPCHK(c, hipStreamSynchronize(c->stream), "sync batched");
+  if (!uneven) {
+    const int64_t blocks = (nq + wpb - 1) / wpb;
+    const int grid = p->grid > 0 ? p->grid : (int)std::min<int64_t>(blocks, 1 << 20);
+    HIPCHK(c, rte::launch_search(a, beam_max, grid, wpb, c->stream), "search launch");
+  } else {
+    // persistent launch: waves pull queries from a counter, longest beams
+    // first (the order array), so mixed beams balance across the CUs
+    HIPCHK(c, c->b_k.reserve(nq * sizeof(int32_t)), "hipMalloc(k_per_q)");
+    HIPCHK(c, c->b_beam.reserve(nq * sizeof(int32_t)), "hipMalloc(beam_per_q)");
+    HIPCHK(c, c->b_order.reserve(nq * sizeof(int32_t)), "hipMalloc(order)");
+    HIPCHK(c, c->b_counter.reserve(sizeof(int64_t)), "hipMalloc(counter)");
+    std::vector<int32_t> order(nq);
+    for (int64_t i = 0; i < nq; ++i) order[i] = (int32_t)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t x, int32_t y) { return hb[x] > hb[y]; });
+    HIPCHK(c, hipMemcpyAsync(c->b_k.p, hk.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice,
+                             c->stream), "copy k_per_q");
+    HIPCHK(c, hipMemcpyAsync(c->b_beam.p, hb.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice,
+                             c->stream), "copy beam_per_q");
+    HIPCHK(c, hipMemcpyAsync(c->b_order.p, order.data(), nq * sizeof(int32_t),
+                             hipMemcpyHostToDevice, c->stream), "copy order");
+    a.k_per_q = c->b_k.as<int32_t>();
+    a.beam_per_q = c->b_beam.as<int32_t>();
+    a.order = c->b_order.as<int32_t>();
+    a.work_counter = c->b_counter.as<int64_t>();
+    int grid = p->grid;
+    if (grid == 0) {
+      int occ = 0, cus = 0;
+      HIPCHK(c, rte::search_occupancy(a, beam_max, wpb, &occ), "occupancy");
+      HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device),
+             "CU count");
+      grid = std::max(1, occ) * std::max(1, cus);
+    }
+    if (int r = launch_batched_graph(c, a, beam_max, grid, wpb)) return r;
+    // the host copies of k/beam/order must outlive the uploads
+    HIPCHK(c, hipStreamSynchronize(c->stream), "sync batched");
+  }
   if (int r = finish_out(c, ids_out, dids, (size_t)nq * a.k_stride, where)) return r;
   if (dists_out)
     if (int r = finish_out(c, dists_out, dd, (size_t)nq * a.k_stride, where)) return r;
@@ -549,6 +617,71 @@ int rte_search_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride
   if (!k_per_q || !beam_per_q) return fail(c, RTE_EINVAL, "k_per_q and beam_per_q required");
   return search_common(c, PQ, nq, q_stride, leaf, k_per_q, beam_per_q, p, k_stride, ids_out,
                        dists_out, nullptr, where);
+}
+
+int rte_autotune_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride,
+                         const int32_t* leaf, const int32_t* k_per_q,
+                         const int32_t* beam_per_q, const rte_params* p, int k_stride,
+                         int32_t* ids_out, float* dists_out, const int* bloom_words,
+                         int n_bloom, rte_params* best, float* best_ms) {
+  if (!c) return RTE_EINVAL;
+  if (!p || !best || !best_ms || !bloom_words || n_bloom < 1)
+    return fail(c, RTE_EINVAL, "bad autotune arguments");
+  if (int r = set_device(c)) return r;
+  int cus = 0;
+  HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device),
+         "CU count");
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0), "event");
+  HIPCHK(c, hipEventCreate(&e1), "event");
+  float best_t = 1e30f;
+  rte_params bp = *p;
+  int rc = RTE_OK;
+  for (int bi = 0; bi < n_bloom && rc == RTE_OK; ++bi)
+    for (int wpb : {1, 2, 4})
+      for (int mult : {1, 2}) {
+        rte_params t = *p;
+        t.bloom_words = bloom_words[bi];
+        t.wg_waves = wpb;
+        t.grid = 0;
+        // resident grid for this shape, times mult
+        if ((rc = rte_search_batched(c, PQ, nq, q_stride, leaf, k_per_q, beam_per_q, &t, k_stride,
+                                     ids_out, dists_out, RTE_DEVICE)) != RTE_OK)
+          break;
+        rte::SearchArgs probe{};
+        probe.d = c->d;
+        probe.shift = c->shift;
+        probe.width = t.width;
+        probe.mode = t.mode;
+        int bw = 0, bh = 0;
+        bloom_for(&t, 512, &bw, &bh);
+        probe.bloom_words_max = std::max(bw, 2048);
+        int occ = 1;
+        (void)rte::search_occupancy(probe, 256, wpb, &occ);
+        t.grid = std::max(1, occ) * cus * mult;
+        float ms_sum = 0.f;
+        for (int rep = 0; rep < 4 && rc == RTE_OK; ++rep) {
+          (void)hipEventRecord(e0, c->stream);
+          rc = rte_search_batched(c, PQ, nq, q_stride, leaf, k_per_q, beam_per_q, &t, k_stride,
+                                  ids_out, dists_out, RTE_DEVICE);
+          (void)hipEventRecord(e1, c->stream);
+          (void)hipEventSynchronize(e1);
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (rep > 0) ms_sum += ms;  // the first run builds the graph
+        }
+        const float ms = ms_sum / 3.f;
+        if (rc == RTE_OK && ms < best_t) {
+          best_t = ms;
+          bp = t;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != RTE_OK) return rc;
+  *best = bp;
+  *best_ms = best_t;
+  return RTE_OK;
 }
 
 int rte_query(rte_ctx* c, const float* Q, int64_t nq, const rte_params* p, float* PQ_scratch,

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "rt_entry.h"
 
@@ -73,6 +74,12 @@ struct rte_ctx {
 
   // staging for host-pointer calls
   DevBuf s_in, s_out, s_aux, s_aux2, s_aux3;
+
+  // batched search: per-query shapes, query order, work counter, and the
+  // captured launch (reused while its key is unchanged)
+  DevBuf b_k, b_beam, b_order, b_counter;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<unsigned char> gkey;
 
   rte::Mlp* mlp = nullptr;
   ~rte_ctx();

@@ -440,7 +440,10 @@ void mlp_destroy(Mlp* m) { delete m; }
 
 }  // namespace rte
 
-rte_ctx::~rte_ctx() { rte::mlp_destroy(mlp); }
+rte_ctx::~rte_ctx() {
+  rte::mlp_destroy(mlp);
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+}
 
 namespace {
 

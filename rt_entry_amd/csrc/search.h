@@ -45,12 +45,14 @@ struct SearchArgs {
 };
 
 int search_supported(int beam_max, int m, int d);
-hipError_t launch_search(const SearchArgs& a, int beam_max, int grid, hipStream_t st);
-// occupancy-derived persistent grid for a launch shape (workgroups)
-int search_persistent_grid(const SearchArgs& a, int beam_max);
+// wpb: queries (waves) per workgroup, 1..4
+hipError_t launch_search(const SearchArgs& a, int beam_max, int grid, int wpb, hipStream_t st);
+// resident workgroups per CU for this launch shape
+hipError_t search_occupancy(const SearchArgs& a, int beam_max, int wpb, int* blocks_per_cu);
 
-// per-beam-register-count launchers (one translation unit each)
+// per-beam-register-count launchers (one translation unit each); with occ
+// non-null they report occupancy instead of launching
 template <int NR, int MR>
-hipError_t launch_nm(const SearchArgs& a, int grid, hipStream_t st);
+hipError_t launch_nm(const SearchArgs& a, int grid, int wpb, hipStream_t st, int* occ);
 
 }  // namespace rte

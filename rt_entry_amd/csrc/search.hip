@@ -18,33 +18,31 @@ static int nr_of(int beam_max) {
   return beam_max <= 64 ? 1 : beam_max <= 128 ? 2 : beam_max <= 256 ? 4 : 8;
 }
 
-hipError_t launch_search(const SearchArgs& a, int beam_max, int grid, hipStream_t st) {
+static hipError_t dispatch(const SearchArgs& a, int beam_max, int grid, int wpb, hipStream_t st,
+                           int* occ) {
   const int m = a.width << a.shift;
   const int MRv = m <= 64 ? 1 : 2;
   switch (nr_of(beam_max) * 4 + MRv) {
-    case 1 * 4 + 1: return launch_nm<1, 1>(a, grid, st);
-    case 1 * 4 + 2: return launch_nm<1, 2>(a, grid, st);
-    case 2 * 4 + 1: return launch_nm<2, 1>(a, grid, st);
-    case 2 * 4 + 2: return launch_nm<2, 2>(a, grid, st);
-    case 4 * 4 + 1: return launch_nm<4, 1>(a, grid, st);
-    case 4 * 4 + 2: return launch_nm<4, 2>(a, grid, st);
-    case 8 * 4 + 1: return launch_nm<8, 1>(a, grid, st);
-    case 8 * 4 + 2: return launch_nm<8, 2>(a, grid, st);
+    case 1 * 4 + 1: return launch_nm<1, 1>(a, grid, wpb, st, occ);
+    case 1 * 4 + 2: return launch_nm<1, 2>(a, grid, wpb, st, occ);
+    case 2 * 4 + 1: return launch_nm<2, 1>(a, grid, wpb, st, occ);
+    case 2 * 4 + 2: return launch_nm<2, 2>(a, grid, wpb, st, occ);
+    case 4 * 4 + 1: return launch_nm<4, 1>(a, grid, wpb, st, occ);
+    case 4 * 4 + 2: return launch_nm<4, 2>(a, grid, wpb, st, occ);
+    case 8 * 4 + 1: return launch_nm<8, 1>(a, grid, wpb, st, occ);
+    case 8 * 4 + 2: return launch_nm<8, 2>(a, grid, wpb, st, occ);
   }
   return hipErrorInvalidValue;
 }
 
-int search_persistent_grid(const SearchArgs& a, int beam_max) {
-  // resident workgroups: CUs x blocks per CU reported by the occupancy API
-  // is kernel-specific; a fixed 4 waves/SIMD bound (16 per CU, 4 per
-  // workgroup) keeps the grid resident for every instantiation
-  int dev = 0, cus = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 1024;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-    return 1024;
-  (void)a;
-  (void)beam_max;
-  return cus * 4;
+hipError_t launch_search(const SearchArgs& a, int beam_max, int grid, int wpb, hipStream_t st) {
+  if (wpb < 1 || wpb > 4 || grid < 1) return hipErrorInvalidValue;
+  return dispatch(a, beam_max, grid, wpb, st, nullptr);
+}
+
+hipError_t search_occupancy(const SearchArgs& a, int beam_max, int wpb, int* blocks_per_cu) {
+  if (wpb < 1 || wpb > 4) return hipErrorInvalidValue;
+  return dispatch(a, beam_max, 1, wpb, nullptr, blocks_per_cu);
 }
 
 }  // namespace rte

@@ -477,7 +477,8 @@ __device__ __forceinline__ int64_t next_query(const SearchArgs& a, int64_t prev,
     if (i >= a.nq) return a.nq;
     return a.order ? a.order[i] : i;
   }
-  return prev < 0 ? (int64_t)blockIdx.x * 4 + wave : prev + (int64_t)gridDim.x * 4;
+  const int wpb = blockDim.x >> 6;
+  return prev < 0 ? (int64_t)blockIdx.x * wpb + wave : prev + (int64_t)gridDim.x * wpb;
 }
 
 template <bool TUNED, int NR, int MR, int J>
@@ -701,28 +702,32 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
 }
 
 template <bool TUNED, int NR, int MR, int J>
-hipError_t launch_t(const SearchArgs& a, int grid, hipStream_t st) {
-  const size_t lds = (size_t)4 * (a.bloom_words_max + lds_words_extra<MR>()) * 4;
-  hipLaunchKernelGGL((search_kernel<TUNED, NR, MR, J>), dim3(grid), dim3(256), lds, st, a);
+hipError_t launch_t(const SearchArgs& a, int grid, int wpb, hipStream_t st, int* occ) {
+  const size_t lds = (size_t)wpb * (a.bloom_words_max + lds_words_extra<MR>()) * 4;
+  if (occ) {  // resident workgroups per CU for this shape
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, search_kernel<TUNED, NR, MR, J>,
+                                                        64 * wpb, lds);
+  }
+  hipLaunchKernelGGL((search_kernel<TUNED, NR, MR, J>), dim3(grid), dim3(64 * wpb), lds, st, a);
   return hipGetLastError();
 }
 
 template <int NR, int MR>
-hipError_t launch_nm(const SearchArgs& a, int grid, hipStream_t st) {
+hipError_t launch_nm(const SearchArgs& a, int grid, int wpb, hipStream_t st, int* occ) {
   const int J = (a.d + 31) / 32;
   const bool tuned = a.mode != 0;
-#define RTE_J_CASES(T)                                  \
-  switch (J) {                                          \
-    case 1:                                             \
-    case 2: return launch_t<T, NR, MR, 2>(a, grid, st); \
-    case 3: return launch_t<T, NR, MR, 3>(a, grid, st); \
-    case 4: return launch_t<T, NR, MR, 4>(a, grid, st); \
-    case 5:                                             \
-    case 6:                                             \
-    case 7:                                             \
-    case 8: return launch_t<T, NR, MR, 8>(a, grid, st); \
-    case 30: return launch_t<T, NR, MR, 30>(a, grid, st); \
-    default: return hipErrorInvalidValue;               \
+#define RTE_J_CASES(T)                                                 \
+  switch (J) {                                                         \
+    case 1:                                                            \
+    case 2: return launch_t<T, NR, MR, 2>(a, grid, wpb, st, occ);      \
+    case 3: return launch_t<T, NR, MR, 3>(a, grid, wpb, st, occ);      \
+    case 4: return launch_t<T, NR, MR, 4>(a, grid, wpb, st, occ);      \
+    case 5:                                                            \
+    case 6:                                                            \
+    case 7:                                                            \
+    case 8: return launch_t<T, NR, MR, 8>(a, grid, wpb, st, occ);      \
+    case 30: return launch_t<T, NR, MR, 30>(a, grid, wpb, st, occ);    \
+    default: return hipErrorInvalidValue;                              \
   }
   if (tuned) {
     RTE_J_CASES(true)

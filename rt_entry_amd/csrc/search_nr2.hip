@@ -3,6 +3,8 @@
 #include "search_impl.h"
 
 namespace rte {
-template hipError_t launch_nm<2, 1>(const SearchArgs&, int, hipStream_t);
-template hipError_t launch_nm<2, 2>(const SearchArgs&, int, hipStream_t);
+template hipError_t launch_nm<2, 1>(const SearchArgs&, int, int, hipStream_t,
+                                           int*);
+template hipError_t launch_nm<2, 2>(const SearchArgs&, int, int, hipStream_t,
+                                           int*);
 }  // namespace rte

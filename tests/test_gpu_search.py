@@ -197,3 +197,56 @@ def test_errors_are_codes(ctx, small_workload):
         ctx.search(w.queries[:, :3].copy(), w.hits, make_params(k=10, beam=64))  # q_stride < d
     with pytest.raises(RteError):  # hash count is fixed per mode
         ctx.search(w.queries, w.hits, make_params(k=10, beam=64, bloom_hashes=5))
+
+
+def test_batched_graph_replays_and_autotune(ctx, int_workload):
+    """The batched launch is a cached hipGraph: replay it several times with
+    eager searches in between and check every replay; then the autotuner's
+    configuration must give the same results."""
+    import torch
+    from rt_entry_amd import data as D
+    from rt_entry_amd import make_params
+    w = int_workload
+    _ctx_for(ctx, w)
+    dev = torch.device("cuda:0")
+    nq = w.queries.shape[0]
+    kq, bq = D.k_beam_assignment(3, nq)
+    po = O.make_params(k=10, beam=64, mode=0)
+    io, do = O.search_uneven(w.base, w.graph, w.queries, w.pids, w.hits, kq, bq, po)
+    PQ = torch.from_numpy(w.queries).to(dev)
+    leaf = torch.from_numpy(w.hits).to(dev)
+    kqd = torch.from_numpy(kq).to(dev)
+    bqd = torch.from_numpy(bq).to(dev)
+    p = make_params(mode=0)
+
+    def check(ids, dists, what):
+        ids = ids.cpu().numpy()
+        dists = dists.cpu().numpy()
+        for q in range(nq):
+            k = kq[q]
+            assert_same(ids[q, :k], io[q, :k], f"{what} ids q={q}")
+            assert_same(dists[q, :k].view(np.int32), do[q, :k].view(np.int32), f"{what} q={q}")
+
+    for rep in range(4):
+        ids, dists = ctx.search_batched(PQ, leaf, kqd, bqd, p, k_stride=io.shape[1])
+        ctx.synchronize()
+        check(ids, dists, f"replay {rep}")
+        ctx.search(PQ, leaf, make_params(k=10, beam=64))  # eager work in between
+        ctx.synchronize()
+    best, ms = ctx.autotune_batched(PQ, leaf, kqd, bqd, p, k_stride=io.shape[1], bloom_words=(0,))
+    print(f"[autotune] wg_waves={best.wg_waves} grid={best.grid} bloom={best.bloom_words} "
+          f"{ms:.3f} ms", flush=True)
+    assert best.wg_waves in (1, 2, 4) and best.grid > 0 and ms > 0
+    ids, dists = ctx.search_batched(PQ, leaf, kqd, bqd, best, k_stride=io.shape[1])
+    ctx.synchronize()
+    check(ids, dists, "autotuned")
+
+
+@pytest.mark.parametrize("wpb", [1, 2, 3])
+def test_search_launch_shapes_same_results(ctx, small_workload, wpb):
+    from rt_entry_amd import make_params
+    w = small_workload
+    _ctx_for(ctx, w)
+    ref, _ = ctx.search(w.queries, w.hits, make_params(k=10, beam=64))
+    ids, _ = ctx.search(w.queries, w.hits, make_params(k=10, beam=64, wg_waves=wpb, grid=7))
+    assert_same(ids, ref, f"wg_waves={wpb}")
