@@ -126,11 +126,14 @@ int rte_select_entry(rte_ctx* ctx, const float* PQ, int64_t nq, int stride,
 int rte_search(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stride,
                const int32_t* leaf, const rte_params* p, int32_t* ids_out,
                float* dists_out, int32_t* stats_out, int where);
-/* Batched uneven search: per-query k and beam (beam rounded up to a power
- * of two and to at least k), one persistent launch with a work queue.
- * Outputs have row stride k_stride >= max k.  No reference counterpart; it
+/* Batched uneven search: per-query k and beam, one persistent launch with a
+ * device work queue (longest beams first), captured as a hipGraph on first
+ * use and replayed while the launch shape is unchanged; no host sync.  Per
+ * query, on the device: k is clamped into [1, k_stride], beam becomes
+ * pow2ceil(max(beam, k, 32)) capped at p->beam (0: 512), and k <= beam.
+ * Outputs have row stride k_stride.  No reference counterpart; it
  * generalises GraphSearchKernel's launch (graph.cu:320-323) to mixed k and
- * beam.  p->k and p->beam are ignored. */
+ * beam.  p->k is ignored. */
 int rte_search_batched(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stride,
                        const int32_t* leaf, const int32_t* k_per_q,
                        const int32_t* beam_per_q, const rte_params* p, int k_stride,

@@ -215,11 +215,18 @@ class Context:
         return tuple(res) if len(res) > 1 else ids
 
     def search_batched(self, PQ, leaf, k_per_q, beam_per_q, params: Params, k_stride=None,
-                       out=None):
+                       out=None, beam_bound=None):
+        """beam_bound: the largest effective beam of the batch (selects the
+        kernel); default: computed from beam_per_q and k_per_q."""
         pq, w, keep = _ptr(PQ, np.float32)
         nq, qs = PQ.shape
         if k_stride is None:
             k_stride = int(k_per_q.max())
+        if beam_bound is None:
+            beam_bound = max(int(beam_per_q.max()), int(k_per_q.max()), 32)
+        bb = 1 << (int(beam_bound) - 1).bit_length()
+        params = Params(*[getattr(params, f) for f, _ in Params._fields_])
+        params.beam = bb
         if out is None:
             if w == RTE_DEVICE:
                 import torch
@@ -312,6 +319,9 @@ class Context:
             k_stride = int(k_per_q.max())
         ids = torch.empty((nq, k_stride), dtype=torch.int32, device=PQ.device)
         dists = torch.empty((nq, k_stride), dtype=torch.float32, device=PQ.device)
+        if params.beam <= 0 or params.beam < int(beam_per_q.max()):
+            params = Params(*[getattr(params, f) for f, _ in Params._fields_])
+            params.beam = 1 << (max(int(beam_per_q.max()), int(k_per_q.max()), 32) - 1).bit_length()
         bw = (C.c_int * len(bloom_words))(*[int(x) for x in bloom_words])
         best = Params()
         ms = C.c_float()

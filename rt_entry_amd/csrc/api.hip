@@ -400,22 +400,71 @@ int rte_select_entry(rte_ctx* c, const float* PQ, int64_t nq, int stride, int32_
   return finish_out(c, leaf_out, dl, (size_t)nq, where);
 }
 
-__global__ void reset_counter_kernel(int64_t* counter) {
-  // one thread; the address depends on threadIdx so it is a vector store
-  counter[threadIdx.x] = 0;
+struct BatchPrep {
+  const int32_t* k_in;
+  const int32_t* beam_in;
+  int64_t nq;
+  int k_stride;
+  int beam_cap;
+};
+
+// counter[0] = work counter, counter[1..5] = per-beam-class counts, then
+// counter[8..12] = scatter cursors; one thread, vector stores (the address
+// depends on threadIdx)
+__global__ void batch_reset_kernel(int64_t* counter) {
+  for (int i = 0; i < 16; ++i) counter[threadIdx.x + i] = 0;
 }
 
-// Counter reset + persistent search kernel, captured once per launch shape as
-// a hipGraph and replayed (the reset is a kernel node, not a memset node).
-static int launch_batched_graph(rte_ctx* c, const rte::SearchArgs& a, int beam_max, int grid,
-                                int wpb) {
-  std::vector<unsigned char> key(sizeof(a) + 4 * sizeof(int) + sizeof(void*));
+// k into [1, min(k_stride, beam)], beam = pow2ceil(max(beam, k, 32)) capped;
+// counts per beam class (32,64,...,512 -> 0..4)
+__global__ void batch_prep_kernel(BatchPrep b, int32_t* __restrict__ k_out,
+                                  int32_t* __restrict__ beam_out, int64_t* counter) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= b.nq) return;
+  int k = b.k_in[q];
+  k = k < 1 ? 1 : (k > b.k_stride ? b.k_stride : k);
+  int bm = b.beam_in[q];
+  bm = bm > k ? bm : k;
+  bm = bm > 32 ? bm : 32;
+  int p2 = 32;
+  while (p2 < bm && p2 < b.beam_cap) p2 <<= 1;
+  if (k > p2) k = p2;
+  k_out[q] = k;
+  beam_out[q] = p2;
+  atomicAdd(reinterpret_cast<unsigned long long*>(counter + 1 + (31 - __clz(p2)) - 5), 1ull);
+}
+
+// cursors: classes in descending beam order
+__global__ void batch_offsets_kernel(int64_t* counter) {
+  int64_t off = 0;
+  for (int cl = 4; cl >= 0; --cl) {
+    counter[threadIdx.x + 8 + cl] = off;
+    off += counter[threadIdx.x + 1 + cl];
+  }
+}
+
+__global__ void batch_order_kernel(const int32_t* __restrict__ beam, int64_t nq,
+                                   int64_t* counter, int32_t* __restrict__ order) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const int cl = (31 - __clz(beam[q])) - 5;
+  const unsigned long long pos =
+      atomicAdd(reinterpret_cast<unsigned long long*>(counter + 8 + cl), 1ull);
+  order[pos] = (int32_t)q;
+}
+
+// Batch preparation + persistent search, captured once per launch shape as a
+// hipGraph and replayed (counters are cleared by a kernel, not a memset).
+static int launch_batched_graph(rte_ctx* c, const rte::SearchArgs& a, const BatchPrep& bp,
+                                int beam_max, int grid, int wpb) {
+  std::vector<unsigned char> key(sizeof(a) + sizeof(bp) + 4 * sizeof(int) + sizeof(void*));
   unsigned char* k = key.data();
   memcpy(k, &a, sizeof(a));
+  memcpy(k + sizeof(a), &bp, sizeof(bp));
   const int extra[4] = {beam_max, grid, wpb, c->device};
-  memcpy(k + sizeof(a), extra, sizeof(extra));
+  memcpy(k + sizeof(a) + sizeof(bp), extra, sizeof(extra));
   void* sp = (void*)c->stream;
-  memcpy(k + sizeof(a) + sizeof(extra), &sp, sizeof(sp));
+  memcpy(k + sizeof(a) + sizeof(bp) + sizeof(extra), &sp, sizeof(sp));
   if (!c->gexec || key != c->gkey) {
     if (c->gexec) {
       (void)hipGraphExecDestroy(c->gexec);
@@ -423,7 +472,13 @@ static int launch_batched_graph(rte_ctx* c, const rte::SearchArgs& a, int beam_m
     }
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed), "begin capture");
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, a.work_counter);
+    const unsigned nb = (unsigned)((bp.nq + 255) / 256);
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, c->stream, a.work_counter);
+    hipLaunchKernelGGL(batch_prep_kernel, dim3(nb), dim3(256), 0, c->stream, bp,
+                       (int32_t*)a.k_per_q, (int32_t*)a.beam_per_q, a.work_counter);
+    hipLaunchKernelGGL(batch_offsets_kernel, dim3(1), dim3(1), 0, c->stream, a.work_counter);
+    hipLaunchKernelGGL(batch_order_kernel, dim3(nb), dim3(256), 0, c->stream, a.beam_per_q,
+                       bp.nq, a.work_counter, (int32_t*)a.order);
     const hipError_t e1 = hipGetLastError();
     const hipError_t e2 = rte::launch_search(a, beam_max, grid, wpb, c->stream);
     const hipError_t e3 = hipStreamEndCapture(c->stream, &g);
@@ -493,23 +548,12 @@ static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
     a.bloom_hashes = h;
     a.bloom_words_max = w;
   } else {
-    // per-query shapes: validate on the host copy
-    hk.resize(nq);
-    hb.resize(nq);
-    const hipMemcpyKind kind = where == RTE_HOST ? hipMemcpyHostToHost : hipMemcpyDeviceToHost;
-    HIPCHK(c, hipMemcpy(hk.data(), k_per_q, nq * sizeof(int32_t), kind), "read k_per_q");
-    HIPCHK(c, hipMemcpy(hb.data(), beam_per_q, nq * sizeof(int32_t), kind), "read beam_per_q");
-    beam_max = 32;
-    int kmax = 1;
-    for (int64_t i = 0; i < nq; ++i) {
-      if (hk[i] < 1 || hb[i] < 1 || hk[i] > k_stride)
-        return fail(c, RTE_EINVAL, "query %lld: k=%d beam=%d k_stride=%d", (long long)i, hk[i],
-                    hb[i], k_stride);
-      hb[i] = pow2_ceil(std::max(std::max(hb[i], hk[i]), 32));
-      if (hb[i] > 512) return fail(c, RTE_EINVAL, "query %lld: beam above 512", (long long)i);
-      beam_max = std::max(beam_max, hb[i]);
-      kmax = std::max(kmax, hk[i]);
-    }
+    // per-query shapes are clamped on the device (batch_prep_kernel):
+    // k into [1, k_stride], beam to pow2ceil(max(beam, k, 32)) <= the bound
+    if (k_stride < 1) return fail(c, RTE_EINVAL, "k_stride < 1");
+    beam_max = p->beam > 0 ? pow2_ceil(std::max(p->beam, 32)) : 512;
+    if (beam_max > 512) return fail(c, RTE_EINVAL, "beam bound %d above 512", p->beam);
+    const int kmax = std::min(k_stride, beam_max);
     a.k_stride = k_stride;
     a.beam = beam_max;
     a.k = kmax;
@@ -562,25 +606,20 @@ static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
     HIPCHK(c, rte::launch_search(a, beam_max, grid, wpb, c->stream), "search launch");
   } else {
     // persistent launch: waves pull queries from a counter, longest beams
-    // first (the order array), so mixed beams balance across the CUs
+    // first (order built on the device), so mixed beams balance across CUs
     HIPCHK(c, c->b_k.reserve(nq * sizeof(int32_t)), "hipMalloc(k_per_q)");
     HIPCHK(c, c->b_beam.reserve(nq * sizeof(int32_t)), "hipMalloc(beam_per_q)");
     HIPCHK(c, c->b_order.reserve(nq * sizeof(int32_t)), "hipMalloc(order)");
-    HIPCHK(c, c->b_counter.reserve(sizeof(int64_t)), "hipMalloc(counter)");
-    std::vector<int32_t> order(nq);
-    for (int64_t i = 0; i < nq; ++i) order[i] = (int32_t)i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t x, int32_t y) { return hb[x] > hb[y]; });
-    HIPCHK(c, hipMemcpyAsync(c->b_k.p, hk.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice,
-                             c->stream), "copy k_per_q");
-    HIPCHK(c, hipMemcpyAsync(c->b_beam.p, hb.data(), nq * sizeof(int32_t), hipMemcpyHostToDevice,
-                             c->stream), "copy beam_per_q");
-    HIPCHK(c, hipMemcpyAsync(c->b_order.p, order.data(), nq * sizeof(int32_t),
-                             hipMemcpyHostToDevice, c->stream), "copy order");
+    HIPCHK(c, c->b_counter.reserve(16 * sizeof(int64_t)), "hipMalloc(counter)");
+    const int32_t* kin;
+    const int32_t* bin;
+    if (int r = stage_in(c, c->b_kin, k_per_q, (size_t)nq, where, &kin)) return r;
+    if (int r = stage_in(c, c->b_bin, beam_per_q, (size_t)nq, where, &bin)) return r;
     a.k_per_q = c->b_k.as<int32_t>();
     a.beam_per_q = c->b_beam.as<int32_t>();
     a.order = c->b_order.as<int32_t>();
     a.work_counter = c->b_counter.as<int64_t>();
+    BatchPrep bp{kin, bin, nq, k_stride, beam_max};
     int grid = p->grid;
     if (grid == 0) {
       int occ = 0, cus = 0;
@@ -589,9 +628,7 @@ static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
              "CU count");
       grid = std::max(1, occ) * std::max(1, cus);
     }
-    if (int r = launch_batched_graph(c, a, beam_max, grid, wpb)) return r;
-    // the host copies of k/beam/order must outlive the uploads
-    HIPCHK(c, hipStreamSynchronize(c->stream), "sync batched");
+    if (int r = launch_batched_graph(c, a, bp, beam_max, grid, wpb)) return r;
   }
   if (int r = finish_out(c, ids_out, dids, (size_t)nq * a.k_stride, where)) return r;
   if (dists_out)

@@ -77,7 +77,7 @@ struct rte_ctx {
 
   // batched search: per-query shapes, query order, work counter, and the
   // captured launch (reused while its key is unchanged)
-  DevBuf b_k, b_beam, b_order, b_counter;
+  DevBuf b_k, b_beam, b_order, b_counter, b_kin, b_bin;
   hipGraphExec_t gexec = nullptr;
   std::vector<unsigned char> gkey;
 
