@@ -129,8 +129,9 @@ def main():
     ap.add_argument("--mode", type=int, default=0, help="0 REF comparator, 1 TUNED")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--n-train", type=int, default=100_000, help="0 skips the learned entry")
-    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--n-train", type=int, default=200_000, help="0 skips the learned entry")
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--min-conf", type=float, default=0.5)
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
@@ -191,25 +192,44 @@ def main():
     # ---- plain method: kd-tree entry
     best_kd, sweep_kd = sweep_and_time(step, recall_of, beams, iters, args.mode, k, "kd-tree")
 
-    # ---- learned entry: labels from the plain search, GPU training
+    # ---- learned entry: labels from the plain search, GPU training with
+    #      early stopping on held-out end-to-end recall (kd-tree fallback on)
     best_ln, sweep_ln, pred_info = None, None, None
     if args.n_train:
         t0 = time.time()
-        PQt = ctx.project(W["train_raw"])
-        kd_t = ctx.select_entry(PQt)
+        n_val = max(1000, args.n_train // 20)
+        PQall = ctx.project(W["train_raw"])
+        kd_all = ctx.select_entry(PQall)
         torch.cuda.synchronize()
-        kd_h = kd_t.cpu().numpy()
-        labels, _ = P.make_labels(ctx, PQt, None, W["gt_train"], beam=64, mode=args.mode,
+        PQt, PQv = PQall[n_val:].contiguous(), PQall[:n_val].contiguous()
+        kd_h = kd_all[n_val:].cpu().numpy()
+        kd_v = kd_all[:n_val].contiguous()
+        gt_t, gt_v = W["gt_train"][n_val:], W["gt_train"][:n_val]
+        labels, _ = P.make_labels(ctx, PQt, None, gt_t, beam=64, mode=args.mode,
                                   kd=kd_h, n_leaf=W["n_leaf"])
         t_lab = time.time() - t0
         log(f"labels in {t_lab:.1f}s; {(labels != kd_h).mean():.3f} differ from the kd leaf")
+        leaf_v = torch.empty(n_val, dtype=torch.int32, device=dev)
+        pv = make_params(k=k, beam=256, max_iter=30, mode=args.mode)
+
+        def validate():
+            ctx.mlp_predict(PQv, fallback=kd_v, min_conf=args.min_conf, out=leaf_v)
+            ids_v = ctx.search(PQv, leaf_v, pv, with_dists=False)
+            torch.cuda.synchronize()
+            return O.recall(ids_v.cpu().numpy(), gt_v, 10)
+
         t0 = time.time()
-        losses = P.train(ctx, PQt, torch.from_numpy(labels).to(dev), W["n_leaf"],
-                         epochs=args.epochs, batch=1024, lr=2e-3,
-                         log=lambda m: log("train " + m) if args.verbose else None)
+        losses, hist = P.train(ctx, PQt, torch.from_numpy(labels).to(dev), W["n_leaf"],
+                               epochs=args.epochs, batch=1024, lr=args.lr, validate=validate,
+                               log=(lambda m: log("train " + m)) if args.verbose else None)
         torch.cuda.synchronize()
         t_train = time.time() - t0
-        log(f"trained {args.epochs} epochs in {t_train:.1f}s, loss {losses[-1]:.3f}")
+        best_ep = max(hist, key=lambda x: x[1])
+        log(f"trained in {t_train:.1f}s; best validation recall {best_ep[1]:.4f} at epoch "
+            f"{best_ep[0]}")
+        temp, temp_hist = P.fit_temperature(ctx, validate)
+        log(f"softmax scale {temp}: validation recall {dict(temp_hist)[temp]:.4f} "
+            f"({temp_hist})")
         use_mlp[0] = True
         conf = torch.empty(nq, dtype=torch.float32, device=dev)
         ctx.project(W["Q"], out=pq)
@@ -217,19 +237,42 @@ def main():
         ctx.mlp_predict(pq, fallback=hits, min_conf=args.min_conf, out=learned, conf=conf)
         torch.cuda.synchronize()
         c = conf.cpu().numpy()
-        pred_info = {"train_queries": args.n_train, "epochs": args.epochs,
-                     "final_loss": round(losses[-1], 4), "label_s": round(t_lab, 1),
-                     "train_s": round(t_train, 1), "min_conf": args.min_conf,
+        pred_info = {"train_queries": int(PQt.shape[0]), "validation_queries": n_val,
+                     "epochs_run": args.epochs, "best_epoch": best_ep[0],
+                     "validation_recall@10_beam256_it30": round(best_ep[1], 4),
+                     "softmax_scale": temp,
+                     "validation_recall_after_scale": round(dict(temp_hist)[temp], 4),
+                     "label_s": round(t_lab, 1), "train_s": round(t_train, 1),
+                     "min_conf": args.min_conf,
                      "fallback_frac": round(float((c < args.min_conf).mean()), 4),
                      "labels_not_kd_leaf": round(float((labels != kd_h).mean()), 4)}
         best_ln, sweep_ln = sweep_and_time(step, recall_of, beams, iters, args.mode, k,
                                            "learned")
 
-    def measure(use, best):
+    def measure(use, best, batched=False):
+        """Timed steps at one setting.  batched: the search runs as the
+        persistent, graph-captured batched launch with an autotuned shape."""
         use_mlp[0] = use
         p = make_params(k=k, beam=best[1], max_iter=best[2], mode=args.mode)
+        kq = torch.full((nq,), k, dtype=torch.int32, device=dev)
+        bq = torch.full((nq,), best[1], dtype=torch.int32, device=dev)
+        tuned = None
+        if batched:
+            step(p)  # entries for the tuning batch
+            entry0 = learned if use else hits
+            tuned, tms = ctx.autotune_batched(pq, entry0, kq, bq, p, k_stride=k)
+            log(f"autotune: wg_waves={tuned.wg_waves} grid={tuned.grid} {tms:.3f} ms")
+
+        def run_search(entry, stats=None):
+            if batched:
+                ctx.search_batched(pq, entry, kq, bq, tuned, k_stride=k, out=(ids, dists),
+                                   beam_bound=best[1])
+            else:
+                ctx.search(pq, entry, p, out=(ids, dists, stats))
+
         for _ in range(args.warmup):
             step(p)
+            run_search(learned if use else hits)
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
               for _ in range(args.steps)]
@@ -242,7 +285,7 @@ def main():
                 ctx.mlp_predict(pq, fallback=hits, min_conf=args.min_conf, out=learned)
                 entry = learned
             ev[i][0].record(stream)
-            ctx.search(pq, entry, p, out=(ids, dists, None))
+            run_search(entry)
             ev[i][1].record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
@@ -265,14 +308,18 @@ def main():
         alg = float((4 * W["d"] * (st[:, 0] + 1) + 4 * D * st[:, 1] + 4 * best[1] + 8 * k).sum())
         return dict(qps=nq * args.steps / wall, ms=wall / args.steps * 1e3, recall=rec,
                     search_ms=search_ms, alg_bytes=alg, p=p, entry=entry, beam=best[1],
-                    max_iter=best[2], iters=float(st[:, 2].mean()), pred_ms=pred_ms)
+                    max_iter=best[2], iters=float(st[:, 2].mean()), pred_ms=pred_ms,
+                    launch=(f"batched wg_waves={tuned.wg_waves} grid={tuned.grid}" if batched
+                            else "grid"))
 
     kd = measure(False, best_kd)
     log(f"kd-tree entry: {kd['qps']:.0f} QPS recall {kd['recall']:.4f}")
+    kd_b = measure(False, best_kd, batched=True)
+    log(f"kd-tree entry, batched launch: {kd_b['qps']:.0f} QPS recall {kd_b['recall']:.4f}")
     ln = None
     if best_ln is not None:
-        ln = measure(True, best_ln)
-        log(f"learned entry: {ln['qps']:.0f} QPS recall {ln['recall']:.4f}")
+        ln = measure(True, best_ln, batched=True)
+        log(f"learned entry, batched launch: {ln['qps']:.0f} QPS recall {ln['recall']:.4f}")
     head = ln if (ln is not None and ln["recall"] >= 0.95) else kd
 
     traffic = None
@@ -329,7 +376,8 @@ def main():
         return {"qps": round(r["qps"], 1), "recall@10": round(r["recall"], 4), "beam": r["beam"],
                 "max_iter": r["max_iter"], "ms_per_step": round(r["ms"], 4),
                 "search_kernel_ms": round(r["search_ms"], 4), "mean_iterations": round(r["iters"], 2),
-                "predictor_ms": round(r["pred_ms"], 4) if r["pred_ms"] is not None else None}
+                "predictor_ms": round(r["pred_ms"], 4) if r["pred_ms"] is not None else None,
+                "launch": r["launch"]}
 
     out = {
         "metric": METRIC,
@@ -348,10 +396,12 @@ def main():
             "workload": f"{W['n']}x{W['d']} integer-valued mixture (workload {args.workload}), "
                         f"{nq} queries, degree-{W['cfg']['degree']} kNN+reverse graph, "
                         f"{W['n_leaf']} kd-tree leaves, k={k}",
-            "entry": "learned (kd-tree fallback)" if head is ln else "kd-tree",
+            "entry": ("learned (kd-tree fallback), batched launch" if head is ln
+                      else "kd-tree"),
             "width": 1, "mode": "tuned" if args.mode else "ref",
             "kd_tree_entry": summary(kd),
-            "learned_entry": summary(ln),
+            "kd_tree_entry_batched": summary(kd_b),
+            "learned_entry_batched": summary(ln),
             "learned_vs_kd_qps": round(ln["qps"] / kd["qps"], 3) if ln else None,
             "predictor": pred_info,
             "sweep_kd_tree": sweep_kd,

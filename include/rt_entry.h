@@ -140,7 +140,8 @@ int rte_search_batched(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stri
                        int32_t* ids_out, float* dists_out, int where);
 
 /* Autotuner for rte_search_batched: times the batch (device buffers) under
- * every launch shape (wg_waves 1/2/4 x persistent grid x1/x2) and every
+ * every launch shape (wg_waves 1/2/4 x grid: one workgroup per wg_waves
+ * queries, or the resident maximum x1/x2) and every
  * filter size in bloom_words[0..n_bloom) (0 = the mode's default), and
  * writes the fastest configuration to *best (a copy of *p with wg_waves,
  * grid and bloom_words set) and its time to *best_ms.  Launch shape never

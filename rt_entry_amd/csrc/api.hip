@@ -416,22 +416,33 @@ __global__ void batch_reset_kernel(int64_t* counter) {
 }
 
 // k into [1, min(k_stride, beam)], beam = pow2ceil(max(beam, k, 32)) capped;
-// counts per beam class (32,64,...,512 -> 0..4)
+// counts per beam class (32,64,...,512 -> 0..4), one atomic per wave and
+// class (same-address device atomics serialise)
 __global__ void batch_prep_kernel(BatchPrep b, int32_t* __restrict__ k_out,
                                   int32_t* __restrict__ beam_out, int64_t* counter) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= b.nq) return;
-  int k = b.k_in[q];
-  k = k < 1 ? 1 : (k > b.k_stride ? b.k_stride : k);
-  int bm = b.beam_in[q];
-  bm = bm > k ? bm : k;
-  bm = bm > 32 ? bm : 32;
-  int p2 = 32;
-  while (p2 < bm && p2 < b.beam_cap) p2 <<= 1;
-  if (k > p2) k = p2;
-  k_out[q] = k;
-  beam_out[q] = p2;
-  atomicAdd(reinterpret_cast<unsigned long long*>(counter + 1 + (31 - __clz(p2)) - 5), 1ull);
+  const bool in = q < b.nq;
+  int cl = -1;
+  if (in) {
+    int k = b.k_in[q];
+    k = k < 1 ? 1 : (k > b.k_stride ? b.k_stride : k);
+    int bm = b.beam_in[q];
+    bm = bm > k ? bm : k;
+    bm = bm > 32 ? bm : 32;
+    int p2 = 32;
+    while (p2 < bm && p2 < b.beam_cap) p2 <<= 1;
+    if (k > p2) k = p2;
+    k_out[q] = k;
+    beam_out[q] = p2;
+    cl = (31 - __clz(p2)) - 5;
+  }
+  const int lane = threadIdx.x & 63;
+  for (int c = 0; c < 5; ++c) {
+    const uint64_t m = __ballot(cl == c);
+    if (m && lane == __ffsll((unsigned long long)m) - 1)
+      atomicAdd(reinterpret_cast<unsigned long long*>(counter + 1 + c),
+                (unsigned long long)__popcll(m));
+  }
 }
 
 // cursors: classes in descending beam order
@@ -443,14 +454,28 @@ __global__ void batch_offsets_kernel(int64_t* counter) {
   }
 }
 
+// scatter query ids by class; one atomic per wave and class, lanes take
+// consecutive slots by their rank in the wave
 __global__ void batch_order_kernel(const int32_t* __restrict__ beam, int64_t nq,
                                    int64_t* counter, int32_t* __restrict__ order) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nq) return;
-  const int cl = (31 - __clz(beam[q])) - 5;
-  const unsigned long long pos =
-      atomicAdd(reinterpret_cast<unsigned long long*>(counter + 8 + cl), 1ull);
-  order[pos] = (int32_t)q;
+  const int cl = q < nq ? (31 - __clz(beam[q])) - 5 : -1;
+  const int lane = threadIdx.x & 63;
+  for (int c = 0; c < 5; ++c) {
+    const uint64_t m = __ballot(cl == c);
+    if (!m) continue;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader)
+      base = atomicAdd(reinterpret_cast<unsigned long long*>(counter + 8 + c),
+                       (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (cl == c) {
+      const int rank = (int)__builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      order[base + rank] = (int32_t)q;
+    }
+  }
 }
 
 // Batch preparation + persistent search, captured once per launch shape as a
@@ -676,7 +701,7 @@ int rte_autotune_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
   int rc = RTE_OK;
   for (int bi = 0; bi < n_bloom && rc == RTE_OK; ++bi)
     for (int wpb : {1, 2, 4})
-      for (int mult : {1, 2}) {
+      for (int mult : {0, 1, 2}) {  // 0: one workgroup per wpb queries
         rte_params t = *p;
         t.bloom_words = bloom_words[bi];
         t.wg_waves = wpb;
@@ -695,7 +720,8 @@ int rte_autotune_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
         probe.bloom_words_max = std::max(bw, 2048);
         int occ = 1;
         (void)rte::search_occupancy(probe, 256, wpb, &occ);
-        t.grid = std::max(1, occ) * cus * mult;
+        t.grid = mult == 0 ? (int)std::min<int64_t>((nq + wpb - 1) / wpb, 1 << 20)
+                           : std::max(1, occ) * cus * mult;
         float ms_sum = 0.f;
         for (int rep = 0; rep < 4 && rc == RTE_OK; ++rep) {
           (void)hipEventRecord(e0, c->stream);

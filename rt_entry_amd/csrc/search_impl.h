@@ -466,19 +466,31 @@ __device__ __forceinline__ uint32_t scan_chunk(const uint64_t (&bf)[NR], const u
   return vf | zm;
 }
 
-// next query of this wave: a shared work counter (persistent launch) or a
-// grid stride
-__device__ __forceinline__ int64_t next_query(const SearchArgs& a, int64_t prev, int wave,
+// Next query of this wave.  Without a work counter: a grid stride.  With one
+// (persistent launch): the first query of every wave is its global wave id
+// (no atomic), later ones come from the counter, which counts past the
+// statically assigned prefix; either way through the order array.
+__device__ __forceinline__ int64_t next_query(const SearchArgs& a, int64_t prev_slot, int wave,
                                               int lane) {
-  if (a.work_counter) {
-    int64_t i = 0;
-    if (lane == 0) i = atomicAdd(reinterpret_cast<unsigned long long*>(a.work_counter), 1ull);
-    i = __builtin_amdgcn_readfirstlane((int)i);
-    if (i >= a.nq) return a.nq;
-    return a.order ? a.order[i] : i;
-  }
   const int wpb = blockDim.x >> 6;
-  return prev < 0 ? (int64_t)blockIdx.x * wpb + wave : prev + (int64_t)gridDim.x * wpb;
+  const int64_t waves = (int64_t)gridDim.x * wpb;
+  int64_t slot;
+  if (!a.work_counter) {
+    slot = prev_slot < 0 ? (int64_t)blockIdx.x * wpb + wave : prev_slot + waves;
+  } else if (prev_slot < 0) {
+    slot = (int64_t)blockIdx.x * wpb + wave;
+  } else if (waves >= a.nq) {
+    slot = a.nq;  // every query was assigned statically
+  } else {
+    unsigned long long i = 0;
+    if (lane == 0) i = atomicAdd(reinterpret_cast<unsigned long long*>(a.work_counter), 1ull);
+    slot = waves + (int64_t)__builtin_amdgcn_readfirstlane((int)i);
+  }
+  return slot;
+}
+
+__device__ __forceinline__ int64_t query_of(const SearchArgs& a, int64_t slot) {
+  return a.order ? a.order[slot] : slot;
 }
 
 template <bool TUNED, int NR, int MR, int J>
@@ -496,7 +508,9 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
   const int64_t np = a.np;
   const int32_t npi = (int32_t)np;
 
-  for (int64_t q = next_query(a, -1, wave, lane); q < a.nq; q = next_query(a, q, wave, lane)) {
+  for (int64_t slot = next_query(a, -1, wave, lane); slot < a.nq;
+       slot = next_query(a, slot, wave, lane)) {
+    const int64_t q = query_of(a, slot);
     // per-query shape (uneven batches carry their own k and beam)
     int n = a.beam, k = a.k;
     if (a.beam_per_q) {

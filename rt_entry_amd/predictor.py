@@ -109,10 +109,17 @@ def make_labels(ctx: Context, PQ, cand, gt, beam: int = 64, width: int = 1,
 
 
 def train(ctx: Context, X, y, n_out: int, hidden=(256, 256), epochs: int = 20,
-          batch: int = 1024, lr: float = 2e-3, seed: int = 1, log=None):
+          batch: int = 1024, lr: float = 2e-3, seed: int = 1, log=None, validate=None,
+          every: int = 2):
     """Trains the MLP on device tensors X (n x d) and labels y (n).  The
     input is standardised per feature with the training set's mean and
-    standard deviation (kept in the network)."""
+    standard deviation (kept in the network).
+
+    validate: optional callable returning a score (higher is better) of the
+    current network, e.g. end-to-end recall with the kd-tree fallback on held
+    out queries; it runs every `every` epochs and the best weights are kept
+    (early stopping on the quantity that matters, not on the training loss).
+    Returns (losses per epoch, validation history [(epoch, score)])."""
     import torch
     n, d = X.shape
     ctx.mlp_create(d, hidden[0], hidden[1], n_out, seed=seed)
@@ -122,21 +129,64 @@ def train(ctx: Context, X, y, n_out: int, hidden=(256, 256), epochs: int = 20,
     ctx.mlp_set_input_norm(mean.float().cpu().numpy(), (1.0 / std).float().cpu().numpy())
     g = torch.Generator(device="cpu")
     g.manual_seed(seed)
-    losses = []
+    losses, history = [], []
+    best = None
     for ep in range(epochs):
         perm = torch.randperm(n, generator=g).to(X.device)
         Xs, ys = X[perm].contiguous(), y[perm].contiguous()
         last = None
         for s in range(0, n - batch + 1, batch):
-            want = log is not None or (ep == epochs - 1 and s + 2 * batch > n)
+            want = log is not None or (s + 2 * batch > n)
             l = ctx.mlp_train_step(Xs[s:s + batch], ys[s:s + batch], lr=lr, want_loss=want)
             if l is not None:
                 last = l
         losses.append(last)
-        if log:
+        if validate is not None and ((ep + 1) % every == 0 or ep == epochs - 1):
+            score = float(validate())
+            history.append((ep + 1, score))
+            if best is None or score > best[0]:
+                best = (score, ep + 1, ctx.mlp_get())
+            if log:
+                log(f"epoch {ep} loss {last:.4f} validation {score:.4f}")
+        elif log:
             log(f"epoch {ep} loss {last:.4f}")
-    return losses
+    if best is not None:
+        ctx.mlp_set(*best[2])
+    return losses, history
 
 
 def predict(ctx: Context, X, fallback, min_conf: float = 0.5, conf=None):
     return ctx.mlp_predict(X, fallback=fallback, min_conf=min_conf, conf=conf)
+
+
+def scale_output(ctx: Context, factor: float):
+    """Multiply the last layer's weights and bias by `factor` (a softmax
+    temperature of 1/factor): the arg-max leaf is unchanged up to bf16
+    rounding, only the confidence used by the fallback rule moves."""
+    W, b = ctx.mlp_get()
+    W[2] = W[2] * np.float32(factor)
+    b[2] = b[2] * np.float32(factor)
+    ctx.mlp_set(W, b)
+
+
+def fit_temperature(ctx: Context, validate, factors=(1.0, 1.5, 2.0, 3.0, 4.0, 6.0, 8.0)):
+    """Choose the output scale (inverse softmax temperature) that maximises
+    `validate()` -- end-to-end recall on held-out queries with the fallback
+    rule applied -- and leave the network scaled by it.  Returns (factor,
+    [(factor, score)])."""
+    W0, b0 = ctx.mlp_get()
+    hist = []
+    for f in factors:
+        W = [w.copy() for w in W0]
+        b = [x.copy() for x in b0]
+        W[2] *= np.float32(f)
+        b[2] *= np.float32(f)
+        ctx.mlp_set(W, b)
+        hist.append((f, float(validate())))
+    best = max(hist, key=lambda x: (x[1], -x[0]))
+    W = [w.copy() for w in W0]
+    b = [x.copy() for x in b0]
+    W[2] *= np.float32(best[0])
+    b[2] *= np.float32(best[0])
+    ctx.mlp_set(W, b)
+    return best[0], hist

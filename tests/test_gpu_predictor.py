@@ -42,8 +42,8 @@ def test_predictor_pipeline_and_gain(ctx, int_workload):
     changed = (labels != kd_tr).mean()
     print(f"[pred] labels differing from the kd-tree leaf: {changed:.3f}", flush=True)
     n_leaf = w.pids.shape[0]
-    losses = P.train(ctx, PQ, torch.from_numpy(labels).to(dev), n_leaf, epochs=25, batch=500,
-                     lr=2e-3, log=None)
+    losses, _ = P.train(ctx, PQ, torch.from_numpy(labels).to(dev), n_leaf, epochs=25,
+                        batch=500, lr=2e-3, log=None)
     assert losses[-1] < 3.5, losses
     Qd = torch.from_numpy(w.queries).to(dev)
     kd = torch.from_numpy(w.hits).to(dev)
