@@ -126,139 +126,327 @@ __global__ __launch_bounds__(256) void gemm_bf16(const uint16_t* __restrict__ A,
     }
 }
 
-// Inference GEMM C = A . B^T with A [M][K] and B [N][K] bf16, both
-// k-contiguous (K a multiple of 8).  Workgroup tile 128x128 (four waves of
-// 64x64, 4x4 v_mfma_f32_16x16x16_bf16 tiles), K staged 32 at a time through
-// LDS with 16-byte global loads and 8-byte operand reads.  Epilogues:
-//  RELU16   out16 = bf16(relu(acc + bias[n]))
-//  SOFTMAX  per row and 64-column wave tile: max of acc + bias, its lowest
-//           column and sum exp(z - max), written as partials for
-//           predict_combine (logits are never stored)
-enum FastEpi { FEPI_RELU16 = 0, FEPI_SOFTMAX = 1 };
+// ---- inference (rte_mlp_predict), v_mfma_f32_16x16x32_bf16 on gfx950 ----
+// Operand layout of the 16x16x32 MFMA used here: lane l supplies row (A) or
+// column (B) l % 16 and the 8 consecutive k values 8 * (l / 16) + 0..7 of a
+// 32-wide k step (A and B use the same k assignment, so only the order of
+// the k sum is implied); lane l holds D[4 * (l / 16) + v][l % 16], v = 0..3.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
+__device__ __forceinline__ floatx4 mfma_k32(uint4 a, uint4 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+__device__ __forceinline__ uint32_t pack2(float lo, float hi) {
+  return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+}
+
+// Hidden layers of the predictor for 32 rows per workgroup (four waves; the
+// waves split the output columns, wave w owning 16-column blocks w, w+4, ...):
+//   x0 = bf16((x - mean) * inv_std)           (as input_kernel)  -> LDS
+//   H1 = bf16(relu(x0 W1 + b1))                                  -> LDS
+//   H2 = bf16(relu(H1 W2 + b2))                                  -> global
+// The A operands come from LDS; the W^T fragments of four 32-wide k steps
+// are loaded from global memory before their MFMAs, so one memory latency
+// covers four steps.  CB = 16-column blocks per wave (columns >= d1 / d2 are
+// computed from zero weights and not stored).  Requires d0 % 32 == 0 and
+// d1 % 32 == 0; dynamic LDS 32 * ((d0 + 8) + (d1 + 8)) * 2 bytes.
+template <int CB>
+struct WFrag {
+  uint4 b[4][CB];  // W^T fragments of four 32-wide k steps
+};
+
+// W^T fragments of k steps k0 .. k0 + 127 for this wave's column blocks
+template <int CB>
+__device__ __forceinline__ void load_wfrag(WFrag<CB>& f, const uint16_t* __restrict__ WT, int N,
+                                           int K, int k0, int wave, int r, int g) {
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int kk = k0 + 32 * u + 8 * g;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      const int col = 16 * (wave + 4 * c) + r;
+      f.b[u][c] = make_uint4(0, 0, 0, 0);
+      if (col < N && kk < K) f.b[u][c] = *reinterpret_cast<const uint4*>(WT + (int64_t)col * K + kk);
+    }
+  }
+}
+
+// acc = As[32 x K] . W^T fragments; `first` holds the fragments of k steps
+// 0..3 (loaded by the caller ahead of time), later groups are loaded while
+// the previous group multiplies.
+template <int CB>
+__device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ As, int lda, int K,
+                                             const uint16_t* __restrict__ WT, int N, int wave,
+                                             int r, int g, WFrag<CB>& first,
+                                             floatx4 (&acc)[2][CB]) {
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int c = 0; c < CB; ++c) acc[rb][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+  WFrag<CB> nxt;
+  for (int k0 = 0; k0 < K; k0 += 128) {
+    if (k0 + 128 < K) load_wfrag<CB>(nxt, WT, N, K, k0 + 128, wave, r, g);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int kk = k0 + 32 * u + 8 * g;
+      if (k0 + 32 * u < K) {
+        uint4 a[2];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+          a[rb] = *reinterpret_cast<const uint4*>(&As[(16 * rb + r) * lda + kk]);
+#pragma unroll
+        for (int c = 0; c < CB; ++c)
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb) acc[rb][c] = mfma_k32(a[rb], first.b[u][c], acc[rb][c]);
+      }
+    }
+    if (k0 + 128 < K) first = nxt;
+  }
+}
+
+template <int CB>
+__global__ __launch_bounds__(256) void mlp_hidden_kernel(
+    const float* __restrict__ X, int64_t n, int d0, const float* __restrict__ mean,
+    const float* __restrict__ inv, const uint16_t* __restrict__ W1T,
+    const float* __restrict__ b1, int d1, const uint16_t* __restrict__ W2T,
+    const float* __restrict__ b2, int d2, uint16_t* __restrict__ H2) {
+  extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
+  const int ld0 = d0 + 8, ld1 = d1 + 8;
+  uint16_t* Xs = lds;
+  uint16_t* H1s = lds + 32 * ld0;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int64_t m0 = (int64_t)blockIdx.x * 32;
+  WFrag<CB> w;  // first W1 group, in flight while the input tile is built
+  load_wfrag<CB>(w, W1T, d1, d0, 0, wave, r, g);
+  {  // normalised bf16 input tile: 8 threads per row, float4 each
+    const int row = t >> 3;
+    const bool in = m0 + row < n;
+    const float* xr = X + (m0 + row) * d0;
+    for (int c4 = (t & 7) * 4; c4 < d0; c4 += 32) {
+      uint2 o = make_uint2(0, 0);
+      if (in) {
+        const float4 x = *reinterpret_cast<const float4*>(xr + c4);
+        const float4 u = *reinterpret_cast<const float4*>(mean + c4);
+        const float4 v = *reinterpret_cast<const float4*>(inv + c4);
+        o.x = pack2((x.x - u.x) * v.x, (x.y - u.y) * v.y);
+        o.y = pack2((x.z - u.z) * v.z, (x.w - u.w) * v.w);
+      }
+      *reinterpret_cast<uint2*>(&Xs[row * ld0 + c4]) = o;
+    }
+  }
+  __syncthreads();
+  floatx4 acc[2][CB];
+  hidden_layer<CB>(Xs, ld0, d0, W1T, d1, wave, r, g, w, acc);
+  load_wfrag<CB>(w, W2T, d2, d1, 0, wave, r, g);  // first W2 group, in flight during the H1 store
+#pragma unroll
+  for (int c = 0; c < CB; ++c) {
+    const int col = 16 * (wave + 4 * c) + r;
+    if (col >= d1) continue;
+    const float bc = b1[col];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        H1s[(16 * rb + 4 * g + v) * ld1 + col] = f2bf(fmaxf(acc[rb][c][v] + bc, 0.f));
+  }
+  __syncthreads();
+  hidden_layer<CB>(H1s, ld1, d1, W2T, d2, wave, r, g, w, acc);
+#pragma unroll
+  for (int c = 0; c < CB; ++c) {
+    const int col = 16 * (wave + 4 * c) + r;
+    if (col >= d2) continue;
+    const float bc = b2[col];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t row = m0 + 16 * rb + 4 * g + v;
+        if (row < n) H2[row * d2 + col] = f2bf(fmaxf(acc[rb][c][v] + bc, 0.f));
+      }
+  }
+}
+
+// Softmax partial of one row over a set of columns: maximum logit, its lowest
+// column, and sum exp(z - max).
 struct SoftPart {
   float mx;
   float se;
   int ix;
 };
 
-template <int EPI>
-__global__ __launch_bounds__(256) void gemm_nt_bf16(const uint16_t* __restrict__ A, int lda,
-                                                    const uint16_t* __restrict__ B, int ldb,
-                                                    int M, int N, int K,
-                                                    const float* __restrict__ bias,
-                                                    uint16_t* __restrict__ out16,
-                                                    SoftPart* __restrict__ parts, int n_parts) {
-  constexpr int LDSK = 40;  // 32 + 8 padding (row = 80 bytes)
-  __shared__ __attribute__((aligned(16))) uint16_t As[128 * LDSK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[128 * LDSK];
+__device__ __forceinline__ void soft_merge(float& mx, float& se, int& ix, float om, float os,
+                                           int oi) {
+  if (om > mx || (om == mx && oi < ix)) {
+    const float t = mx;
+    mx = om;
+    ix = oi;
+    om = t;
+    const float u = se;
+    se = os;
+    os = u;
+  }
+  if (om > -INFINITY) se += os * __expf(om - mx);
+}
+
+// Online softmax update of one row with four logits of a tile (columns
+// col0 + 16 j), without branches: running max, lowest argmax column and
+// sum exp(z - max), the exponentials as 2^(z log2(e) - max log2(e)).
+// The running max starts finite (-1e30, below every logit) with se = 0;
+// logits are finite or -inf (padding columns), which contribute nothing.
+__device__ __forceinline__ void soft_update(float& mx, float& se, int& ix, float z0, float z1,
+                                            float z2, float z3, int col0) {
+  constexpr float L2E = 1.4426950408889634f;
+  const float tm = fmaxf(fmaxf(z0, z1), fmaxf(z2, z3));
+  int tj = z2 == tm ? 32 : 48;
+  tj = z1 == tm ? 16 : tj;
+  tj = z0 == tm ? 0 : tj;
+  asm volatile("" : "+v"(tj));  // keep the select below a v_cndmask, not a branch
+  ix = tm > mx ? col0 + tj : ix;
+  const float nm = fmaxf(mx, tm);
+  const float nl = -nm * L2E;
+  se *= __builtin_amdgcn_exp2f(__builtin_fmaf(mx, L2E, nl));
+  mx = nm;
+  se += (__builtin_amdgcn_exp2f(__builtin_fmaf(z0, L2E, nl)) +
+         __builtin_amdgcn_exp2f(__builtin_fmaf(z1, L2E, nl))) +
+        (__builtin_amdgcn_exp2f(__builtin_fmaf(z2, L2E, nl)) +
+         __builtin_amdgcn_exp2f(__builtin_fmaf(z3, L2E, nl)));
+}
+
+// Output layer fused with the softmax reduction, for 128 rows x one group of
+// `cg` columns per workgroup (grid.x = column group, so consecutive
+// workgroups, which land on different XCDs, read disjoint slices of W3).
+// Wave w keeps the bf16 H2 fragments of its 32 rows in registers for the
+// whole group; 64-column tiles of W3^T ([Npad][K] bf16, rows >= N zero,
+// Npad a multiple of 64) are double-buffered through LDS (one barrier per
+// tile).  The accumulators start from the bias, so each logit z = acc feeds
+// an online (max, lowest argmax column, sum exp) per row directly; logits
+// are never stored.  The 16 lanes sharing a row merge their states at the
+// end and write one SoftPart per (row, group).  K = 32 * KS; dynamic LDS
+// 2 * 64 * (K + 8) * 2 bytes; bias has Npad entries, -inf beyond N.
+template <int KS>
+__global__ __launch_bounds__(256, 2) void mlp_head_kernel(const uint16_t* __restrict__ H,
+                                                          int64_t n,
+                                                          const uint16_t* __restrict__ WT,
+                                                          const float* __restrict__ bias, int N,
+                                                          int cg, SoftPart* __restrict__ parts) {
+  constexpr int K = 32 * KS;
+  constexpr int LDB = K + 8;
+  constexpr int CPR = K / 8;  // 16-byte chunks per column
+  extern __shared__ __attribute__((aligned(16))) uint16_t Bs[];
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  const int m0 = blockIdx.y * 128, n0 = blockIdx.x * 128;
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-  floatx4 acc[4][4];
+  const int r = lane & 15, g = lane >> 4;
+  const int grp = blockIdx.x, G = gridDim.x;
+  const int c0 = grp * cg;
+  const int c1 = min(N, c0 + cg);
+  const int64_t m0 = (int64_t)blockIdx.y * 128 + 32 * wave;
+  uint4 A[2][KS];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int rb = 0; rb < 2; ++rb) {
+    const int64_t row = m0 + 16 * rb + r;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 32) {
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int c = t + 256 * e;  // 512 chunks of 8 elements per 128x32 tile
-      const int r = c >> 2, c8 = (c & 3) * 8;
-      const int gk = k0 + c8;
-      uint4 va = make_uint4(0, 0, 0, 0), vb = make_uint4(0, 0, 0, 0);
-      if (m0 + r < M && gk < K)
-        va = *reinterpret_cast<const uint4*>(A + (int64_t)(m0 + r) * lda + gk);
-      if (n0 + r < N && gk < K)
-        vb = *reinterpret_cast<const uint4*>(B + (int64_t)(n0 + r) * ldb + gk);
-      *reinterpret_cast<uint4*>(&As[r * LDSK + c8]) = va;
-      *reinterpret_cast<uint4*>(&Bs[r * LDSK + c8]) = vb;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < 32; kk += 16) {
-      const int kq = kk + 4 * (lane >> 4);
-      short4_t a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        a[i] = *reinterpret_cast<const short4_t*>(&As[(wm + 16 * i + (lane & 15)) * LDSK + kq]);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        b[j] = *reinterpret_cast<const short4_t*>(&Bs[(wn + 16 * j + (lane & 15)) * LDSK + kq]);
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
-    __syncthreads();
+    for (int s = 0; s < KS; ++s)
+      A[rb][s] = row < n ? *reinterpret_cast<const uint4*>(H + row * K + 32 * s + 8 * g)
+                         : make_uint4(0, 0, 0, 0);
   }
-  // D layout: lane holds D[4*(lane/16) + v][lane%16] of each 16x16 tile
-  if constexpr (EPI == FEPI_RELU16) {
+  // tile loader: thread t moves the 16-byte chunks t + 256 i (i < KS) of a
+  // 64-column tile, chunk q = column q / CPR, k offset 8 (q % CPR)
+  uint4 pre[KS];
+  const uint16_t* wt_t = WT + (int64_t)(t / CPR) * K + 8 * (t % CPR);
+  uint16_t* bs_t = Bs + (t / CPR) * LDB + 8 * (t % CPR);
+  constexpr int COLS_PER_STEP = 256 / CPR;  // columns advanced per i
+  float mx[2][4], se[2][4];
+  int ix[2][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+  for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int n = n0 + wn + 16 * j + (lane & 15);
-        if (n >= N) continue;
-        const float bn = bias[n];
+    for (int v = 0; v < 4; ++v) {
+      mx[rb][v] = -1e30f;  // below every logit, finite (see soft_update)
+      se[rb][v] = 0.f;
+      ix[rb][v] = 0x7fffffff;
+    }
+  const int ntile = (c1 - c0 + 63) >> 6;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
-          if (m < M) out16[(int64_t)m * N + n] = f2bf(fmaxf(acc[i][j][v] + bn, 0.f));
-        }
-      }
-  } else {
-    float bn[4];
+  for (int i = 0; i < KS; ++i)
+    pre[i] = *reinterpret_cast<const uint4*>(wt_t + (int64_t)(c0 + COLS_PER_STEP * i) * K);
+#pragma unroll
+  for (int i = 0; i < KS; ++i)
+    *reinterpret_cast<uint4*>(bs_t + COLS_PER_STEP * i * LDB) = pre[i];
+  __syncthreads();
+  for (int tile = 0; tile < ntile; ++tile) {
+    const int n0 = c0 + 64 * tile;
+    const int buf = tile & 1;
+    // next tile (the group's first again after the last: loaded, stored
+    // into the idle buffer and never read)
+    const int nn = tile + 1 < ntile ? n0 + 64 : c0;
+#pragma unroll
+    for (int i = 0; i < KS; ++i)
+      pre[i] = *reinterpret_cast<const uint4*>(wt_t + (int64_t)(nn + COLS_PER_STEP * i) * K);
+    floatx4 acc[2][4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int n = n0 + wn + 16 * j + (lane & 15);
-      bn[j] = n < N ? bias[n] : 0.f;
+      const float bj = bias[n0 + 16 * j + r];  // padded to whole tiles with -inf
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) acc[rb][j] = floatx4{bj, bj, bj, bj};
     }
+    {  // B fragments of k step s + 1 are read from LDS while step s multiplies
+      const uint16_t* bb = &Bs[(buf * 64 + r) * LDB + 8 * g];
+      uint4 bc[4], bn[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) bc[j] = *reinterpret_cast<const uint4*>(bb + 16 * j * LDB);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        // this lane's 4 columns of row m, then the 16 lanes holding the row
-        float mx = -INFINITY;
-        int ix = 0x7fffffff;
+      for (int s = 0; s < KS; ++s) {
+        if (s + 1 < KS) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int n = n0 + wn + 16 * j + (lane & 15);
-          const float z = n < N ? acc[i][j][v] + bn[j] : -INFINITY;
-          if (z > mx) {
-            mx = z;
-            ix = n;
-          }
+          for (int j = 0; j < 4; ++j)
+            bn[j] = *reinterpret_cast<const uint4*>(bb + 16 * j * LDB + 32 * (s + 1));
         }
 #pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-          const float om = __shfl_xor(mx, o, 64);
-          const int oi = __shfl_xor(ix, o, 64);
-          if (om > mx || (om == mx && oi < ix)) {
-            mx = om;
-            ix = oi;
-          }
-        }
-        float se = 0.f;
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int n = n0 + wn + 16 * j + (lane & 15);
-          if (n < N) se += expf(acc[i][j][v] + bn[j] - mx);
-        }
+          for (int rb = 0; rb < 2; ++rb) acc[rb][j] = mfma_k32(A[rb][s], bc[j], acc[rb][j]);
 #pragma unroll
-        for (int o = 1; o < 16; o <<= 1) se += __shfl_xor(se, o, 64);
-        const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
-        if ((lane & 15) == 0 && m < M) {
-          SoftPart p;
-          p.mx = mx;
-          p.se = se;
-          p.ix = ix;
-          parts[(int64_t)m * n_parts + blockIdx.x * 2 + (wn >> 6)] = p;
-        }
+        for (int j = 0; j < 4; ++j) bc[j] = bn[j];
       }
+    }
+    // columns >= N carry a -inf bias (zero weights), so their logits are
+    // -inf and drop out of max and sum
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        soft_update(mx[rb][v], se[rb][v], ix[rb][v], acc[rb][0][v], acc[rb][1][v], acc[rb][2][v],
+                    acc[rb][3][v], n0 + r);
+#pragma unroll
+    for (int i = 0; i < KS; ++i)
+      *reinterpret_cast<uint4*>(bs_t + ((buf ^ 1) * 64 + COLS_PER_STEP * i) * LDB) = pre[i];
+    __syncthreads();
   }
+  // merge the 16 lanes of each row (lanes differing in bits 0..3)
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        const float om = __shfl_xor(mx[rb][v], o, 64);
+        const float os = __shfl_xor(se[rb][v], o, 64);
+        const int oi = __shfl_xor(ix[rb][v], o, 64);
+        soft_merge(mx[rb][v], se[rb][v], ix[rb][v], om, os, oi);
+      }
+      const int64_t row = m0 + 16 * rb + 4 * g + v;
+      if (r == 0 && row < n) {
+        SoftPart p;
+        p.mx = mx[rb][v];
+        p.se = se[rb][v];
+        p.ix = ix[rb][v];
+        parts[row * G + grp] = p;
+      }
+    }
 }
 
 // row-wise combination of the softmax partials: leaf = lowest column of the
@@ -284,13 +472,13 @@ __global__ void predict_combine(const SoftPart* __restrict__ parts, int n_parts,
   leaf[m] = (pr >= min_conf || !fallback) ? gi : fallback[m];
 }
 
-// W [K][N] f32 -> WT [N][K] bf16
-__global__ void transpose_bf16_kernel(const float* __restrict__ W, int K, int N,
+// W [K][N] f32 -> WT [Npad][K] bf16, rows n >= N zero
+__global__ void transpose_bf16_kernel(const float* __restrict__ W, int K, int N, int Npad,
                                       uint16_t* __restrict__ WT) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (int64_t)K * N) return;
+  if (i >= (int64_t)K * Npad) return;
   const int n = (int)(i / K), k = (int)(i % K);
-  WT[i] = f2bf(W[(int64_t)k * N + n]);
+  WT[i] = n < N ? f2bf(W[(int64_t)k * N + n]) : (uint16_t)0;
 }
 
 __global__ void to_bf16_kernel(const float* __restrict__ x, int64_t n, uint16_t* __restrict__ y) {
@@ -429,7 +617,8 @@ struct Mlp {
   DevBuf X16, H1, H1b, H2, H2b, Z, dZ, dZb, dH2, dH2b, dH1, dH1b, loss;
   DevBuf in_mean, in_inv;  // input normalisation (identity by default)
   DevBuf WT16[3];          // transposed bf16 weights for inference
-  DevBuf P1, P2, parts;    // inference activations and softmax partials
+  DevBuf b3p;              // output bias padded to whole 64-column tiles
+  DevBuf P2, parts;        // inference: H2 (bf16) and softmax partials
   bool wt_dirty = true;
   int pcap = 0;
   int cap = 0;
@@ -759,11 +948,15 @@ int rte_mlp_predict(rte_ctx* c, const float* X, int64_t n, const int32_t* fallba
   Mlp* m = c->mlp;
   MCHK(c, hipSetDevice(c->device), "hipSetDevice");
   const int d0 = m->dims[0], d1 = m->dims[1], d2 = m->dims[2], d3 = m->dims[3];
-  const int chunk = 16384;
-  const int b0 = (int)std::min<int64_t>(n, chunk);
-  MCHK(c, reserve_batch(m, b0), "batch buffers");
-  const bool fast = d0 % 8 == 0 && d1 % 8 == 0 && d2 % 8 == 0;
-  if (!fast) {  // generic kernels for shapes the 16-byte path cannot load
+  // fused inference kernels: input and first hidden width multiples of 32
+  // (first hidden <= 512), second hidden width 32 * 2^i <= 512, 16-byte
+  // aligned input rows; other shapes take the training kernels
+  const bool fast = d0 % 32 == 0 && d1 % 32 == 0 && d1 <= 512 &&
+                    (d2 == 32 || d2 == 64 || d2 == 128 || d2 == 256 || d2 == 512) &&
+                    ((uintptr_t)X & 15) == 0;
+  if (!fast) {
+    const int chunk = 16384;
+    MCHK(c, reserve_batch(m, (int)std::min<int64_t>(n, chunk)), "batch buffers");
     for (int64_t s = 0; s < n; s += chunk) {
       const int b = (int)std::min<int64_t>(chunk, n - s);
       MCHK(c, load_input(m, X + s * d0, b, c->stream), "input");
@@ -777,44 +970,87 @@ int rte_mlp_predict(rte_ctx* c, const float* X, int64_t n, const int32_t* fallba
   }
   if (m->wt_dirty) {
     for (int l = 0; l < 3; ++l) {
-      const int64_t nw = wsize(m, l);
+      // the output layer's rows are padded to whole 64-column tiles
+      const int np = l == 2 ? (m->dims[3] + 63) / 64 * 64 : m->dims[l + 1];
+      const int64_t nw = (int64_t)m->dims[l] * np;
       MCHK(c, m->WT16[l].reserve(nw * 2), "transposed weights");
       hipLaunchKernelGGL(rte::transpose_bf16_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256),
-                         0, c->stream, m->W[l].as<float>(), m->dims[l], m->dims[l + 1],
+                         0, c->stream, m->W[l].as<float>(), m->dims[l], m->dims[l + 1], np,
                          m->WT16[l].as<uint16_t>());
     }
     MCHK(c, hipGetLastError(), "transpose weights");
+    const int np = (d3 + 63) / 64 * 64;
+    MCHK(c, m->b3p.reserve((size_t)np * 4), "padded bias");
+    // -inf bias on the padding columns: their logits are -inf
+    MCHK(c, hipMemsetD32Async((hipDeviceptr_t)m->b3p.p, 0xff800000u, (size_t)np, c->stream),
+         "padded bias");
+    MCHK(c, hipMemcpyAsync(m->b3p.p, m->b[2].p, (size_t)d3 * 4, hipMemcpyDeviceToDevice, c->stream),
+         "padded bias");
     m->wt_dirty = false;
   }
-  const int n_parts = 2 * ((d3 + 127) / 128);
+  const int cg = 1024;  // output columns per head workgroup
+  const int G = (d3 + cg - 1) / cg;
+  const int chunk = 65536;
+  const int b0 = (int)std::min<int64_t>(n, chunk);
   if (b0 > m->pcap) {
-    MCHK(c, m->P1.reserve((size_t)b0 * d1 * 2), "P1");
-    MCHK(c, m->P2.reserve((size_t)b0 * d2 * 2), "P2");
-    MCHK(c, m->parts.reserve((size_t)b0 * n_parts * sizeof(rte::SoftPart)), "parts");
+    MCHK(c, m->P2.reserve((size_t)b0 * d2 * 2), "hidden activations");
     m->pcap = b0;
-  } else {
-    MCHK(c, m->parts.reserve((size_t)b0 * n_parts * sizeof(rte::SoftPart)), "parts");
   }
+  MCHK(c, m->parts.reserve((size_t)b0 * G * sizeof(rte::SoftPart)), "softmax partials");
+  const int cbw = std::max((d1 + 63) / 64, (d2 + 63) / 64);
+  const int cb = cbw <= 1 ? 1 : cbw <= 2 ? 2 : cbw <= 4 ? 4 : 8;
+  const int ks = d2 / 32;
+  const size_t lds_h = (size_t)32 * ((d0 + 8) + (d1 + 8)) * 2;
+  const size_t lds_o = (size_t)2 * 64 * (d2 + 8) * 2;
+  const void* hk = cb == 1   ? (const void*)rte::mlp_hidden_kernel<1>
+                   : cb == 2 ? (const void*)rte::mlp_hidden_kernel<2>
+                   : cb == 4 ? (const void*)rte::mlp_hidden_kernel<4>
+                             : (const void*)rte::mlp_hidden_kernel<8>;
+  const void* ok = ks == 1    ? (const void*)rte::mlp_head_kernel<1>
+                   : ks == 2  ? (const void*)rte::mlp_head_kernel<2>
+                   : ks == 4  ? (const void*)rte::mlp_head_kernel<4>
+                   : ks == 8  ? (const void*)rte::mlp_head_kernel<8>
+                              : (const void*)rte::mlp_head_kernel<16>;
+  MCHK(c, hipFuncSetAttribute(ok, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_o),
+       "head kernel LDS size");
+  MCHK(c, hipFuncSetAttribute(hk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_h),
+       "hidden kernel LDS size");
   for (int64_t s = 0; s < n; s += chunk) {
-    const int b = (int)std::min<int64_t>(chunk, n - s);
-    MCHK(c, load_input(m, X + s * d0, b, c->stream), "input");
-    auto grid = [](int M_, int N_) {
-      return dim3((unsigned)((N_ + 127) / 128), (unsigned)((M_ + 127) / 128));
-    };
-    hipLaunchKernelGGL(rte::gemm_nt_bf16<rte::FEPI_RELU16>, grid(b, d1), dim3(256), 0, c->stream,
-                       m->X16.as<uint16_t>(), d0, m->WT16[0].as<uint16_t>(), d0, b, d1, d0,
-                       m->b[0].as<float>(), m->P1.as<uint16_t>(), (rte::SoftPart*)nullptr, 0);
-    hipLaunchKernelGGL(rte::gemm_nt_bf16<rte::FEPI_RELU16>, grid(b, d2), dim3(256), 0, c->stream,
-                       m->P1.as<uint16_t>(), d1, m->WT16[1].as<uint16_t>(), d1, b, d2, d1,
-                       m->b[1].as<float>(), m->P2.as<uint16_t>(), (rte::SoftPart*)nullptr, 0);
-    hipLaunchKernelGGL(rte::gemm_nt_bf16<rte::FEPI_SOFTMAX>, grid(b, d3), dim3(256), 0, c->stream,
-                       m->P2.as<uint16_t>(), d2, m->WT16[2].as<uint16_t>(), d2, b, d3, d2,
-                       m->b[2].as<float>(), (uint16_t*)nullptr, m->parts.as<rte::SoftPart>(),
-                       n_parts);
+    const int64_t b = std::min<int64_t>(chunk, n - s);
+    const float* Xs = X + s * d0;
+    const float* mean = m->in_mean.as<float>();
+    const float* inv = m->in_inv.as<float>();
+    const uint16_t* W1T = m->WT16[0].as<uint16_t>();
+    const uint16_t* W2T = m->WT16[1].as<uint16_t>();
+    const uint16_t* W3T = m->WT16[2].as<uint16_t>();
+    uint16_t* H2 = m->P2.as<uint16_t>();
+    rte::SoftPart* parts = m->parts.as<rte::SoftPart>();
+    const dim3 gh((unsigned)((b + 31) / 32));
+#define HID(CB)                                                                                   \
+  hipLaunchKernelGGL(rte::mlp_hidden_kernel<CB>, gh, dim3(256), lds_h, c->stream, Xs, b, d0, mean, \
+                     inv, W1T, m->b[0].as<float>(), d1, W2T, m->b[1].as<float>(), d2, H2)
+    switch (cb) {
+      case 1: HID(1); break;
+      case 2: HID(2); break;
+      case 4: HID(4); break;
+      default: HID(8); break;
+    }
+#undef HID
+    const dim3 go((unsigned)G, (unsigned)((b + 127) / 128));
+#define HEAD(KS)                                                                              \
+  hipLaunchKernelGGL(rte::mlp_head_kernel<KS>, go, dim3(256), lds_o, c->stream, H2, b, W3T,     \
+                     m->b3p.as<float>(), d3, cg, parts)
+    switch (ks) {
+      case 1: HEAD(1); break;
+      case 2: HEAD(2); break;
+      case 4: HEAD(4); break;
+      case 8: HEAD(8); break;
+      default: HEAD(16); break;
+    }
+#undef HEAD
     hipLaunchKernelGGL(rte::predict_combine, dim3((unsigned)((b + 255) / 256)), dim3(256), 0,
-                       c->stream, m->parts.as<rte::SoftPart>(), n_parts, (int64_t)b,
-                       fallback ? fallback + s : nullptr, min_conf, leaf_out + s,
-                       conf_out ? conf_out + s : nullptr);
+                       c->stream, parts, G, b, fallback ? fallback + s : nullptr, min_conf,
+                       leaf_out + s, conf_out ? conf_out + s : nullptr);
     MCHK(c, hipGetLastError(), "predict");
   }
   return RTE_OK;
