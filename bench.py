@@ -101,6 +101,7 @@ def sweep_and_time(step, recall_of, beams, iters, mode, k, log_name):
 
     sweep, best = [], None
     for beam in beams:
+        lo = 0  # largest budget known to miss the target
         for mi in iters:
             p = make_params(k=k, beam=beam, max_iter=mi, mode=mode)
             rec = recall_of(p)
@@ -108,9 +109,28 @@ def sweep_and_time(step, recall_of, beams, iters, mode, k, log_name):
             sweep.append(dict(beam=beam, max_iter=mi, recall=round(rec, 4), ms=round(t * 1e3, 3)))
             log(f"{log_name} sweep beam={beam} max_iter={mi} recall={rec:.4f} {t * 1e3:.3f} ms")
             if rec >= 0.95:
+                # recall never falls as max_iter grows (the search is a fixed
+                # trajectory cut after max_iter steps): bisect to the first
+                # budget that reaches the target
+                hi = mi
+                while hi - lo > 1:
+                    mid = (lo + hi) // 2
+                    pm = make_params(k=k, beam=beam, max_iter=mid, mode=mode)
+                    rm = recall_of(pm)
+                    log(f"{log_name} refine beam={beam} max_iter={mid} recall={rm:.4f}")
+                    if rm >= 0.95:
+                        hi, rec = mid, rm
+                    else:
+                        lo = mid
+                if hi != mi:
+                    p = make_params(k=k, beam=beam, max_iter=hi, mode=mode)
+                    t = time_steps(p, 2)
+                    sweep.append(dict(beam=beam, max_iter=hi, recall=round(rec, 4),
+                                      ms=round(t * 1e3, 3)))
                 if best is None or t < best[0]:
-                    best = (t, beam, mi, rec)
+                    best = (t, beam, hi, rec)
                 break
+            lo = mi
     if best is None:
         s = max(sweep, key=lambda x: x["recall"])
         best = (s["ms"] / 1e3, s["beam"], s["max_iter"], s["recall"])
@@ -126,13 +146,24 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--beam", type=int, default=0, help="skip the sweeps: fixed beam")
     ap.add_argument("--max-iter", type=int, default=0)
+    ap.add_argument("--kd", default="", help="BEAM,MAX_ITER: fixed kd-tree setting (no sweep)")
+    ap.add_argument("--learned", default="", help="BEAM,MAX_ITER: fixed learned setting (no sweep)")
+    ap.add_argument("--only", choices=("all", "kd", "learned"), default="all",
+                    help="measure only one entry method (profiling runs)")
+    ap.add_argument("--save-predictor", default="", help="write the trained network (.npz)")
+    ap.add_argument("--load-predictor", default="", help="use a saved network, no training")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after timing, 5 device-to-device copies of this many MiB (a "
+                         "known-size kernel for scaling the HBM byte counters)")
     ap.add_argument("--mode", type=int, default=0, help="0 REF comparator, 1 TUNED")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--n-train", type=int, default=200_000, help="0 skips the learned entry")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--min-conf", type=float, default=0.5)
+    ap.add_argument("--min-conf", type=float, default=None,
+                    help="kd-tree fallback below this confidence (default: fitted on "
+                         "held-out queries from 0.1, 0.2, 0.3, 0.5)")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
 
@@ -171,14 +202,17 @@ def main():
     dists = torch.empty((nq, k), dtype=torch.float32, device=dev)
     use_mlp = [False]
 
+    def entries(use):
+        """Entry leaves of the projected queries: kd-tree, or the network
+        with kd-tree fallback (one call; the kd-tree walk overlaps the
+        network on a second stream)."""
+        if use:
+            return ctx.predict_entry(pq, min_conf=args.min_conf, out=learned)
+        return ctx.select_entry(pq, out=hits)
+
     def step(p):
         ctx.project(W["Q"], out=pq)
-        ctx.select_entry(pq, out=hits)
-        entry = hits
-        if use_mlp[0]:
-            ctx.mlp_predict(pq, fallback=hits, min_conf=args.min_conf, out=learned)
-            entry = learned
-        ctx.search(pq, entry, p, out=(ids, dists, None))
+        ctx.search(pq, entries(use_mlp[0]), p, out=(ids, dists, None))
 
     def recall_of(p):
         step(p)
@@ -186,16 +220,34 @@ def main():
         return O.recall(ids.cpu().numpy(), W["gt"], 10)
 
     beams = (args.beam,) if args.beam else (32, 64, 128, 256)
-    iters = (args.max_iter or 100,) if args.beam else (10, 15, 20, 25, 30, 40, 50, 60, 80, 100,
-                                                       150, 200)
+    iters = (args.max_iter or 100,) if args.beam else (10, 15, 20, 25, 30, 35, 40, 45, 50, 55, 60,
+                                                       70, 80, 100, 150, 200)
+
+    def setting(spec, name):
+        """Fixed (beam, max_iter) from 'B,I', or the sweep."""
+        if spec:
+            b_, i_ = (int(x) for x in spec.split(","))
+            p = make_params(k=k, beam=b_, max_iter=i_, mode=args.mode)
+            rec = recall_of(p)
+            log(f"{name} fixed beam={b_} max_iter={i_} recall={rec:.4f}")
+            return (None, b_, i_, rec), [dict(beam=b_, max_iter=i_, recall=round(rec, 4))]
+        return sweep_and_time(step, recall_of, beams, iters, args.mode, k, name)
 
     # ---- plain method: kd-tree entry
-    best_kd, sweep_kd = sweep_and_time(step, recall_of, beams, iters, args.mode, k, "kd-tree")
+    best_kd, sweep_kd = None, None
+    if args.only != "learned":
+        best_kd, sweep_kd = setting(args.kd, "kd-tree")
 
     # ---- learned entry: labels from the plain search, GPU training with
     #      early stopping on held-out end-to-end recall (kd-tree fallback on)
     best_ln, sweep_ln, pred_info = None, None, None
-    if args.n_train:
+    if args.only != "kd" and args.load_predictor:
+        extra = P.load(ctx, args.load_predictor)
+        pred_info = {"loaded": os.path.basename(args.load_predictor), **extra}
+        if args.min_conf is None:
+            args.min_conf = float(extra.get("min_conf", 0.5))
+        log(f"predictor loaded from {args.load_predictor}: {extra}")
+    elif args.only != "kd" and args.n_train:
         t0 = time.time()
         n_val = max(1000, args.n_train // 20)
         PQall = ctx.project(W["train_raw"])
@@ -205,21 +257,28 @@ def main():
         kd_h = kd_all[n_val:].cpu().numpy()
         kd_v = kd_all[:n_val].contiguous()
         gt_t, gt_v = W["gt_train"][n_val:], W["gt_train"][:n_val]
-        labels, _ = P.make_labels(ctx, PQt, None, gt_t, beam=64, mode=args.mode,
-                                  kd=kd_h, n_leaf=W["n_leaf"])
+        # the three best-converging leaves of every training query (scored by
+        # the plain search from every leaf), each a training pair: softer
+        # targets than the single best leaf, which is often a near tie
+        lab3, _ = P.make_labels(ctx, PQt, None, gt_t, beam=64, mode=args.mode,
+                                kd=kd_h, n_leaf=W["n_leaf"], top=3)
+        labels = lab3[:, 0]
         t_lab = time.time() - t0
         log(f"labels in {t_lab:.1f}s; {(labels != kd_h).mean():.3f} differ from the kd leaf")
+        X3 = PQt.repeat(3, 1).contiguous()
+        y3 = torch.from_numpy(np.ascontiguousarray(lab3.T.reshape(-1))).to(dev)
         leaf_v = torch.empty(n_val, dtype=torch.int32, device=dev)
-        pv = make_params(k=k, beam=256, max_iter=30, mode=args.mode)
+        pv = make_params(k=k, beam=128, max_iter=32, mode=args.mode)
+        min_conf = [0.5 if args.min_conf is None else args.min_conf]
 
         def validate():
-            ctx.mlp_predict(PQv, fallback=kd_v, min_conf=args.min_conf, out=leaf_v)
+            ctx.mlp_predict(PQv, fallback=kd_v, min_conf=min_conf[0], out=leaf_v)
             ids_v = ctx.search(PQv, leaf_v, pv, with_dists=False)
             torch.cuda.synchronize()
             return O.recall(ids_v.cpu().numpy(), gt_v, 10)
 
         t0 = time.time()
-        losses, hist = P.train(ctx, PQt, torch.from_numpy(labels).to(dev), W["n_leaf"],
+        losses, hist = P.train(ctx, X3, y3, W["n_leaf"],
                                epochs=args.epochs, batch=1024, lr=args.lr, validate=validate,
                                log=(lambda m: log("train " + m)) if args.verbose else None)
         torch.cuda.synchronize()
@@ -230,6 +289,26 @@ def main():
         temp, temp_hist = P.fit_temperature(ctx, validate)
         log(f"softmax scale {temp}: validation recall {dict(temp_hist)[temp]:.4f} "
             f"({temp_hist})")
+        mc_hist = []
+        if args.min_conf is None:  # fallback threshold, on the same held-out queries
+            for mc in (0.1, 0.2, 0.3, 0.5):
+                min_conf[0] = mc
+                mc_hist.append((mc, validate()))
+            min_conf[0] = max(mc_hist, key=lambda x: (x[1], x[0]))[0]
+            log(f"fallback threshold {min_conf[0]} ({mc_hist})")
+        args.min_conf = min_conf[0]
+        pred_info = {"train_queries": int(PQt.shape[0]), "labels_per_query": 3,
+                     "validation_queries": n_val,
+                     "epochs_run": args.epochs, "best_epoch": best_ep[0],
+                     "validation_recall@10_beam128_it32": round(best_ep[1], 4),
+                     "softmax_scale": temp,
+                     "validation_recall_after_scale": round(dict(temp_hist)[temp], 4),
+                     "min_conf_fit": [(a, round(b, 4)) for a, b in mc_hist],
+                     "label_s": round(t_lab, 1), "train_s": round(t_train, 1),
+                     "labels_not_kd_leaf": round(float((labels != kd_h).mean()), 4)}
+        if args.save_predictor:
+            P.save(ctx, args.save_predictor, softmax_scale=temp, min_conf=args.min_conf)
+    if pred_info is not None:
         use_mlp[0] = True
         conf = torch.empty(nq, dtype=torch.float32, device=dev)
         ctx.project(W["Q"], out=pq)
@@ -237,53 +316,40 @@ def main():
         ctx.mlp_predict(pq, fallback=hits, min_conf=args.min_conf, out=learned, conf=conf)
         torch.cuda.synchronize()
         c = conf.cpu().numpy()
-        pred_info = {"train_queries": int(PQt.shape[0]), "validation_queries": n_val,
-                     "epochs_run": args.epochs, "best_epoch": best_ep[0],
-                     "validation_recall@10_beam256_it30": round(best_ep[1], 4),
-                     "softmax_scale": temp,
-                     "validation_recall_after_scale": round(dict(temp_hist)[temp], 4),
-                     "label_s": round(t_lab, 1), "train_s": round(t_train, 1),
-                     "min_conf": args.min_conf,
-                     "fallback_frac": round(float((c < args.min_conf).mean()), 4),
-                     "labels_not_kd_leaf": round(float((labels != kd_h).mean()), 4)}
-        best_ln, sweep_ln = sweep_and_time(step, recall_of, beams, iters, args.mode, k,
-                                           "learned")
+        pred_info.update({"min_conf": args.min_conf,
+                          "fallback_frac": round(float((c < args.min_conf).mean()), 4)})
+        best_ln, sweep_ln = setting(args.learned, "learned")
 
     def measure(use, best, batched=False):
         """Timed steps at one setting.  batched: the search runs as the
         persistent, graph-captured batched launch with an autotuned shape."""
         use_mlp[0] = use
         p = make_params(k=k, beam=best[1], max_iter=best[2], mode=args.mode)
-        kq = torch.full((nq,), k, dtype=torch.int32, device=dev)
-        bq = torch.full((nq,), best[1], dtype=torch.int32, device=dev)
         tuned = None
         if batched:
-            step(p)  # entries for the tuning batch
-            entry0 = learned if use else hits
-            tuned, tms = ctx.autotune_batched(pq, entry0, kq, bq, p, k_stride=k)
+            # every query has the same k and beam: a uniform batch (the graph
+            # holds the search kernel alone)
+            ctx.project(W["Q"], out=pq)
+            entry0 = entries(use)  # entries for the tuning batch
+            tuned, tms = ctx.autotune_batched(pq, entry0, None, None, p, k_stride=k)
             log(f"autotune: wg_waves={tuned.wg_waves} grid={tuned.grid} {tms:.3f} ms")
 
         def run_search(entry, stats=None):
             if batched:
-                ctx.search_batched(pq, entry, kq, bq, tuned, k_stride=k, out=(ids, dists),
-                                   beam_bound=best[1])
+                ctx.search_batched(pq, entry, None, None, tuned, k_stride=k, out=(ids, dists))
             else:
                 ctx.search(pq, entry, p, out=(ids, dists, stats))
 
         for _ in range(args.warmup):
-            step(p)
-            run_search(learned if use else hits)
+            ctx.project(W["Q"], out=pq)
+            run_search(entries(use))
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
               for _ in range(args.steps)]
         t0 = time.perf_counter()
         for i in range(args.steps):
             ctx.project(W["Q"], out=pq)
-            ctx.select_entry(pq, out=hits)
-            entry = hits
-            if use:
-                ctx.mlp_predict(pq, fallback=hits, min_conf=args.min_conf, out=learned)
-                entry = learned
+            entry = entries(use)
             ev[i][0].record(stream)
             run_search(entry)
             ev[i][1].record(stream)
@@ -296,7 +362,7 @@ def main():
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
             for _ in range(args.steps):
-                ctx.mlp_predict(pq, fallback=hits, min_conf=args.min_conf, out=learned)
+                entries(True)
             e1.record(stream)
             torch.cuda.synchronize()
             pred_ms = e0.elapsed_time(e1) / args.steps
@@ -312,25 +378,41 @@ def main():
                     launch=(f"batched wg_waves={tuned.wg_waves} grid={tuned.grid}" if batched
                             else "grid"))
 
-    kd = measure(False, best_kd)
-    log(f"kd-tree entry: {kd['qps']:.0f} QPS recall {kd['recall']:.4f}")
-    kd_b = measure(False, best_kd, batched=True)
-    log(f"kd-tree entry, batched launch: {kd_b['qps']:.0f} QPS recall {kd_b['recall']:.4f}")
+    kd = kd_b = None
+    if best_kd is not None:
+        kd = measure(False, best_kd)
+        log(f"kd-tree entry: {kd['qps']:.0f} QPS recall {kd['recall']:.4f}")
+        kd_b = measure(False, best_kd, batched=True)
+        log(f"kd-tree entry, batched launch: {kd_b['qps']:.0f} QPS recall {kd_b['recall']:.4f}")
     ln = None
     if best_ln is not None:
         ln = measure(True, best_ln, batched=True)
         log(f"learned entry, batched launch: {ln['qps']:.0f} QPS recall {ln['recall']:.4f}")
-    head = ln if (ln is not None and ln["recall"] >= 0.95) else kd
+    head = ln if (ln is not None and (ln["recall"] >= 0.95 or kd is None)) else kd
 
+    if args.copy_mib:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters give
+        # the scale of the counters (bytes per counted KiB) for the search kernel
+        a = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 5 x {args.copy_mib} MiB device copies")
+        del a, b
+
+    # HBM bytes per launch of this setting, from rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs scaled by a copy of known size (null if not profiled)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "search_traffic.json")
     if os.path.exists(tf):
-        try:
-            tj = json.load(open(tf))
-            if tj.get("beam") == head["beam"] and tj.get("max_iter") == head["max_iter"]:
+        entry = "learned" if head is ln else "kd-tree"
+        for tj in json.load(open(tf)):
+            if (tj.get("beam"), tj.get("max_iter"), tj.get("entry")) == (
+                    head["beam"], head["max_iter"], entry) and args.workload == 2 and \
+                    args.scale == 1.0:
                 traffic = tj.get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
     achieved = head["alg_bytes"] / (head["search_ms"] * 1e-3)
 
     # ---- CPU baseline: the oracle's search on the host cores, bounded sample
@@ -338,9 +420,7 @@ def main():
     if not args.no_cpu_baseline:
         use_mlp[0] = head is ln
         ctx.project(W["Q"], out=pq)
-        ctx.select_entry(pq, out=hits)
-        if use_mlp[0]:
-            ctx.mlp_predict(pq, fallback=hits, min_conf=args.min_conf, out=learned)
+        entries(use_mlp[0])
         torch.cuda.synchronize()
         pq_h = pq.cpu().numpy()
         ent_h = (learned if use_mlp[0] else hits).cpu().numpy()
@@ -402,7 +482,9 @@ def main():
             "kd_tree_entry": summary(kd),
             "kd_tree_entry_batched": summary(kd_b),
             "learned_entry_batched": summary(ln),
-            "learned_vs_kd_qps": round(ln["qps"] / kd["qps"], 3) if ln else None,
+            "learned_vs_kd_qps": round(ln["qps"] / kd["qps"], 3) if ln and kd else None,
+            "learned_vs_kd_batched_qps": (round(ln["qps"] / kd_b["qps"], 3) if ln and kd_b
+                                          else None),
             "predictor": pred_info,
             "sweep_kd_tree": sweep_kd,
             "sweep_learned": sweep_ln,
@@ -410,7 +492,9 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
-            "kernel": "search_kernel", "algorithmic_bytes_per_launch": head["alg_bytes"],
+            "kernel": "search_kernel", "launch": head["launch"],
+            "kernel_ms": round(head["search_ms"], 4),
+            "algorithmic_bytes_per_launch": head["alg_bytes"],
         },
         "cpu_baseline": cpu,
     }

@@ -133,7 +133,11 @@ int rte_search(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stride,
  * pow2ceil(max(beam, k, 32)) capped at p->beam (0: 512), and k <= beam.
  * Outputs have row stride k_stride.  No reference counterpart; it
  * generalises GraphSearchKernel's launch (graph.cu:320-323) to mixed k and
- * beam.  p->k is ignored. */
+ * beam.  p->k is ignored.
+ * With k_per_q = beam_per_q = NULL the batch is uniform (every query uses
+ * p->k and p->beam, k_stride >= p->k): the captured graph is the search
+ * kernel alone, its waves walking the queries grid-stride over p->grid
+ * workgroups (0: the resident maximum); results equal rte_search's. */
 int rte_search_batched(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stride,
                        const int32_t* leaf, const int32_t* k_per_q,
                        const int32_t* beam_per_q, const rte_params* p, int k_stride,
@@ -162,8 +166,9 @@ int rte_query(rte_ctx* ctx, const float* Q, int64_t nq, const rte_params* p,
 /* ---- learned entry predictor (no reference counterpart) ---------------- */
 /* MLP d_in -> h1 -> h2 -> n_leaf, ReLU, bf16 weights, FP32 accumulation. */
 int rte_mlp_create(rte_ctx* ctx, int d_in, int h1, int h2, int n_out, uint64_t seed);
-/* One SGD-momentum step on a device batch (X: n x d_in f32, y: n int32
- * labels); writes the mean cross-entropy loss to *loss (host). */
+/* One Adam step (beta1 = momentum, beta2 = 0.999) on a device batch (X: n x
+ * d_in f32, y: n int32 labels); writes the mean cross-entropy loss to *loss
+ * (host) when loss is not NULL. */
 int rte_mlp_train_step(rte_ctx* ctx, const float* X, const int32_t* y, int n, float lr,
                        float momentum, float* loss);
 /* Predicted leaf and softmax confidence per query (device buffers).
@@ -171,6 +176,15 @@ int rte_mlp_train_step(rte_ctx* ctx, const float* X, const int32_t* y, int n, fl
  * softmax maximum is below min_conf. */
 int rte_mlp_predict(rte_ctx* ctx, const float* X, int64_t n, const int32_t* fallback,
                     float min_conf, int32_t* leaf_out, float* conf_out);
+/* Entry leaves of nq PCA queries (device buffers, row stride `stride` >=
+ * max(3, d_in)): the network's leaf, or the kd-tree leaf of rte_select_entry
+ * when the softmax maximum is below min_conf.  Same results as
+ * rte_select_entry followed by rte_mlp_predict; the kd-tree traversal runs
+ * on a second stream of the context, overlapping the network, and is
+ * joined before the fallback is applied.  Needs rte_set_leaves and
+ * rte_mlp_create. */
+int rte_predict_entry(rte_ctx* ctx, const float* PQ, int64_t nq, int stride, float min_conf,
+                      int32_t* leaf_out, float* conf_out);
 /* Input normalisation x' = (x - mean) * inv_std per feature (host arrays of
  * d_in), applied in FP32 before the bf16 rounding; identity until set. */
 int rte_mlp_set_input_norm(rte_ctx* ctx, const float* mean, const float* inv_std);

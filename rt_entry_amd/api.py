@@ -216,17 +216,23 @@ class Context:
 
     def search_batched(self, PQ, leaf, k_per_q, beam_per_q, params: Params, k_stride=None,
                        out=None, beam_bound=None):
-        """beam_bound: the largest effective beam of the batch (selects the
-        kernel); default: computed from beam_per_q and k_per_q."""
+        """Batched launch (captured hipGraph).  With per-query k_per_q and
+        beam_per_q the batch is uneven; beam_bound is its largest effective
+        beam (selects the kernel; default: computed from the arrays).  With
+        both None the batch is uniform: every query uses params.k and
+        params.beam (results equal search())."""
         pq, w, keep = _ptr(PQ, np.float32)
         nq, qs = PQ.shape
-        if k_stride is None:
-            k_stride = int(k_per_q.max())
-        if beam_bound is None:
-            beam_bound = max(int(beam_per_q.max()), int(k_per_q.max()), 32)
-        bb = 1 << (int(beam_bound) - 1).bit_length()
+        uniform = k_per_q is None and beam_per_q is None
+        if (k_per_q is None) != (beam_per_q is None):
+            raise ValueError("k_per_q and beam_per_q: both or neither")
         params = Params(*[getattr(params, f) for f, _ in Params._fields_])
-        params.beam = bb
+        if k_stride is None:
+            k_stride = params.k if uniform else int(k_per_q.max())
+        if not uniform:
+            if beam_bound is None:
+                beam_bound = max(int(beam_per_q.max()), int(k_per_q.max()), 32)
+            params.beam = 1 << (int(beam_bound) - 1).bit_length()
         if out is None:
             if w == RTE_DEVICE:
                 import torch
@@ -239,10 +245,12 @@ class Context:
         else:
             ids, dists = out
         pl, _, _ = _ptr(leaf, np.int32) if leaf is not None else (None, w, None)
-        pk, wk, k1 = _ptr(k_per_q, np.int32)
-        pb, wb, k2 = _ptr(beam_per_q, np.int32)
-        if wk != w or wb != w:
-            raise ValueError("k_per_q / beam_per_q must live with the queries")
+        pk = pb = None
+        if not uniform:
+            pk, wk, k1 = _ptr(k_per_q, np.int32)
+            pb, wb, k2 = _ptr(beam_per_q, np.int32)
+            if wk != w or wb != w:
+                raise ValueError("k_per_q / beam_per_q must live with the queries")
         pi, _, _ = _ptr(ids, np.int32)
         pd, _, _ = _ptr(dists, np.float32)
         self._chk(lib().rte_search_batched(self._h, pq, nq, qs, pl, pk, pb, C.byref(params),
@@ -260,6 +268,7 @@ class Context:
         v = np.ascontiguousarray(inv_std, np.float32)
         self._chk(lib().rte_mlp_set_input_norm(self._h, m.ctypes.data_as(C.c_void_p),
                                                v.ctypes.data_as(C.c_void_p)))
+        self.mlp_norm = (m.copy(), v.copy())
 
     def mlp_get(self):
         d = self.mlp_dims
@@ -309,17 +318,33 @@ class Context:
                                         float(min_conf), C.c_void_p(out.data_ptr()), cp))
         return out
 
+    def predict_entry(self, PQ, min_conf: float = 0.5, out=None, conf=None):
+        """Entry leaves of device PCA queries: the network's leaf, or the
+        kd-tree leaf below min_conf (the kd-tree selection overlaps the
+        network on a second stream)."""
+        import torch
+        nq, qs = PQ.shape
+        if out is None:
+            out = torch.empty(nq, dtype=torch.int32, device=PQ.device)
+        cp = C.c_void_p(conf.data_ptr()) if conf is not None else None
+        self._chk(lib().rte_predict_entry(self._h, C.c_void_p(PQ.data_ptr()), int(nq),
+                                          int(PQ.stride(0)), float(min_conf),
+                                          C.c_void_p(out.data_ptr()), cp))
+        return out
+
     def autotune_batched(self, PQ, leaf, k_per_q, beam_per_q, params: Params, k_stride=None,
                          bloom_words=(0,)):
         """Fastest (wg_waves, grid, bloom_words) for this batch (device
-        tensors).  Returns (best Params, best ms)."""
+        tensors; k_per_q = beam_per_q = None for a uniform batch).  Returns
+        (best Params, best ms)."""
         import torch
         nq, qs = PQ.shape
+        uniform = k_per_q is None and beam_per_q is None
         if k_stride is None:
-            k_stride = int(k_per_q.max())
+            k_stride = params.k if uniform else int(k_per_q.max())
         ids = torch.empty((nq, k_stride), dtype=torch.int32, device=PQ.device)
         dists = torch.empty((nq, k_stride), dtype=torch.float32, device=PQ.device)
-        if params.beam <= 0 or params.beam < int(beam_per_q.max()):
+        if not uniform and (params.beam <= 0 or params.beam < int(beam_per_q.max())):
             params = Params(*[getattr(params, f) for f, _ in Params._fields_])
             params.beam = 1 << (max(int(beam_per_q.max()), int(k_per_q.max()), 32) - 1).bit_length()
         bw = (C.c_int * len(bloom_words))(*[int(x) for x in bloom_words])

@@ -498,12 +498,14 @@ static int launch_batched_graph(rte_ctx* c, const rte::SearchArgs& a, const Batc
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed), "begin capture");
     const unsigned nb = (unsigned)((bp.nq + 255) / 256);
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, c->stream, a.work_counter);
-    hipLaunchKernelGGL(batch_prep_kernel, dim3(nb), dim3(256), 0, c->stream, bp,
-                       (int32_t*)a.k_per_q, (int32_t*)a.beam_per_q, a.work_counter);
-    hipLaunchKernelGGL(batch_offsets_kernel, dim3(1), dim3(1), 0, c->stream, a.work_counter);
-    hipLaunchKernelGGL(batch_order_kernel, dim3(nb), dim3(256), 0, c->stream, a.beam_per_q,
-                       bp.nq, a.work_counter, (int32_t*)a.order);
+    if (bp.k_in) {  // uneven batch (a uniform one needs no preparation)
+      hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, c->stream, a.work_counter);
+      hipLaunchKernelGGL(batch_prep_kernel, dim3(nb), dim3(256), 0, c->stream, bp,
+                         (int32_t*)a.k_per_q, (int32_t*)a.beam_per_q, a.work_counter);
+      hipLaunchKernelGGL(batch_offsets_kernel, dim3(1), dim3(1), 0, c->stream, a.work_counter);
+      hipLaunchKernelGGL(batch_order_kernel, dim3(nb), dim3(256), 0, c->stream, a.beam_per_q,
+                         bp.nq, a.work_counter, (int32_t*)a.order);
+    }
     const hipError_t e1 = hipGetLastError();
     const hipError_t e2 = rte::launch_search(a, beam_max, grid, wpb, c->stream);
     const hipError_t e3 = hipStreamEndCapture(c->stream, &g);
@@ -526,7 +528,7 @@ static int launch_batched_graph(rte_ctx* c, const rte::SearchArgs& a, const Batc
 static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride,
                          const int32_t* leaf, const int32_t* k_per_q, const int32_t* beam_per_q,
                          const rte_params* p, int k_stride, int32_t* ids_out, float* dists_out,
-                         int32_t* stats_out, int where) {
+                         int32_t* stats_out, int where, bool batched) {
   const bool uneven = beam_per_q != nullptr;
   if (int r = check_params(c, p, uneven)) return r;
   if (!c->base) return fail(c, RTE_ESTATE, "rte_set_base not called");
@@ -566,6 +568,10 @@ static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
     a.beam = beam;
     a.k = p->k;
     a.k_stride = p->k;
+    if (batched) {  // uniform batch: rows of k_stride >= k
+      if (k_stride < p->k) return fail(c, RTE_EINVAL, "k_stride %d < k %d", k_stride, p->k);
+      a.k_stride = k_stride;
+    }
     beam_max = beam;
     int w, h;
     bloom_for(p, beam, &w, &h);
@@ -625,10 +631,23 @@ static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
   a.ids_out = dids;
   a.dists_out = dd;
   a.stats_out = dst;
-  if (!uneven) {
+  if (!batched) {
     const int64_t blocks = (nq + wpb - 1) / wpb;
     const int grid = p->grid > 0 ? p->grid : (int)std::min<int64_t>(blocks, 1 << 20);
     HIPCHK(c, rte::launch_search(a, beam_max, grid, wpb, c->stream), "search launch");
+  } else if (!uneven) {
+    // uniform batch: the captured graph is the search kernel alone; waves
+    // walk the queries grid-stride (no preparation, no work counter)
+    int grid = p->grid;
+    if (grid == 0) {
+      int occ = 0, cus = 0;
+      HIPCHK(c, rte::search_occupancy(a, beam_max, wpb, &occ), "occupancy");
+      HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device),
+             "CU count");
+      grid = std::max(1, occ) * std::max(1, cus);
+    }
+    BatchPrep bp{nullptr, nullptr, nq, k_stride, beam_max};
+    if (int r = launch_batched_graph(c, a, bp, beam_max, grid, wpb)) return r;
   } else {
     // persistent launch: waves pull queries from a counter, longest beams
     // first (order built on the device), so mixed beams balance across CUs
@@ -668,7 +687,7 @@ int rte_search(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride, const 
                int where) {
   if (!c) return RTE_EINVAL;
   return search_common(c, PQ, nq, q_stride, leaf, nullptr, nullptr, p, p ? p->k : 0, ids_out,
-                       dists_out, stats_out, where);
+                       dists_out, stats_out, where, false);
 }
 
 int rte_search_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride,
@@ -676,9 +695,10 @@ int rte_search_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride
                        const rte_params* p, int k_stride, int32_t* ids_out, float* dists_out,
                        int where) {
   if (!c) return RTE_EINVAL;
-  if (!k_per_q || !beam_per_q) return fail(c, RTE_EINVAL, "k_per_q and beam_per_q required");
+  if (!k_per_q != !beam_per_q)
+    return fail(c, RTE_EINVAL, "k_per_q and beam_per_q: both or neither");
   return search_common(c, PQ, nq, q_stride, leaf, k_per_q, beam_per_q, p, k_stride, ids_out,
-                       dists_out, nullptr, where);
+                       dists_out, nullptr, where, true);
 }
 
 int rte_autotune_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride,

@@ -9,6 +9,11 @@
 // index ranges, laid out in preorder with skip links, visits leaves in
 // ascending id order and the first containing leaf is the smallest such id.
 // A miss yields leaf 0 (entry.cu:88-89 fills hits with 0).
+//
+// Memory layout: the nodes are stored breadth-first with first-child and
+// skip links (the walk order stays preorder), so the top levels are a prefix
+// of the array; each workgroup copies that prefix (kTopNodes nodes, 64 KB)
+// into LDS and only the deepest levels are read from global memory.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -21,6 +26,15 @@ __global__ __launch_bounds__(256) void select_entry_kernel(const BoxNode* __rest
                                                            const float* __restrict__ pq,
                                                            int64_t nq, int stride,
                                                            int32_t* __restrict__ hits) {
+  __shared__ BoxNode top[kTopNodes];
+  const int n_top = min(n_nodes, kTopNodes);
+  for (int i = threadIdx.x; i < n_top; i += blockDim.x) {
+    const uint4* src = reinterpret_cast<const uint4*>(nodes + i);
+    uint4* dst = reinterpret_cast<uint4*>(top + i);
+    dst[0] = src[0];
+    dst[1] = src[1];
+  }
+  __syncthreads();
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
   const float x = pq[q * stride + 0];
@@ -29,17 +43,17 @@ __global__ __launch_bounds__(256) void select_entry_kernel(const BoxNode* __rest
   int hit = 0;
   int i = 0;
   while (i < n_nodes) {
-    const BoxNode nd = nodes[i];
+    const BoxNode nd = i < n_top ? top[i] : nodes[i];
     const bool in = x >= nd.lo[0] && x <= nd.hi[0] && y >= nd.lo[1] && y <= nd.hi[1] &&
                     z >= nd.lo[2] && z <= nd.hi[2];
     if (in) {
-      if (nd.leaf >= 0) {
-        hit = nd.leaf;
+      if (nd.link >= 0) {
+        hit = nd.link;
         break;
       }
-      ++i;  // first child follows in preorder
+      i = -nd.link - 1;  // first child
     } else {
-      i = nd.skip;  // skip links always point forward, so the walk ends
+      i = nd.skip;  // later in preorder, so the walk ends
     }
   }
   hits[q] = hit;
@@ -64,7 +78,7 @@ static int build_rec(const float* aabbs, int lo, int hi, std::vector<BoxNode>& o
       n.lo[a] = aabbs[(size_t)lo * 6 + a];
       n.hi[a] = aabbs[(size_t)lo * 6 + 3 + a];
     }
-    n.leaf = lo;
+    n.link = lo;
   } else {
     const int mid = lo + (hi - lo) / 2;
     const int l = build_rec(aabbs, lo, mid, out);
@@ -74,7 +88,7 @@ static int build_rec(const float* aabbs, int lo, int hi, std::vector<BoxNode>& o
       n.lo[a] = fminf(out[l].lo[a], out[r].lo[a]);
       n.hi[a] = fmaxf(out[l].hi[a], out[r].hi[a]);
     }
-    n.leaf = -1;
+    n.link = -1;
   }
   out[me].skip = (int)out.size();
   return me;
@@ -83,8 +97,32 @@ static int build_rec(const float* aabbs, int lo, int hi, std::vector<BoxNode>& o
 void build_hierarchy(const float* aabbs, int n_leaf, std::vector<BoxNode>& out) {
   out.clear();
   if (n_leaf <= 0) return;
-  out.reserve((size_t)2 * n_leaf);
-  build_rec(aabbs, 0, n_leaf, out);
+  std::vector<BoxNode> pre;  // preorder: first child = index + 1, skip links
+  pre.reserve((size_t)2 * n_leaf);
+  build_rec(aabbs, 0, n_leaf, pre);
+  const int n = (int)pre.size();
+  // breadth-first relabelling; children of preorder node p: p + 1 and the
+  // skip of p + 1
+  std::vector<int> bfs;  // bfs position -> preorder index
+  std::vector<int> pos(n + 1, n);  // preorder index -> bfs position (n -> n)
+  bfs.reserve(n);
+  bfs.push_back(0);
+  for (size_t h = 0; h < bfs.size(); ++h) {
+    const int p = bfs[h];
+    pos[p] = (int)h;
+    if (pre[p].link < 0) {
+      bfs.push_back(p + 1);
+      bfs.push_back(pre[p + 1].skip);
+    }
+  }
+  out.resize(n);
+  for (int h = 0; h < n; ++h) {
+    const BoxNode& a = pre[bfs[h]];
+    BoxNode b = a;
+    b.skip = pos[a.skip];
+    b.link = a.link >= 0 ? a.link : -(pos[bfs[h] + 1]) - 1;
+    out[h] = b;
+  }
 }
 
 }  // namespace rte

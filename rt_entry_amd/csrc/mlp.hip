@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "entry.h"
 #include "mlp.h"
 
 namespace rte {
@@ -150,8 +151,9 @@ __device__ __forceinline__ uint32_t pack2(float lo, float hi) {
 // The A operands come from LDS; the W^T fragments of four 32-wide k steps
 // are loaded from global memory before their MFMAs, so one memory latency
 // covers four steps.  CB = 16-column blocks per wave (columns >= d1 / d2 are
-// computed from zero weights and not stored).  Requires d0 % 32 == 0 and
-// d1 % 32 == 0; dynamic LDS 32 * ((d0 + 8) + (d1 + 8)) * 2 bytes.
+// computed from zero weights and not stored).  Input rows are ldx floats
+// apart.  Requires d0 % 32 == 0, ldx % 4 == 0 and d1 % 32 == 0; dynamic LDS
+// 32 * ((d0 + 8) + (d1 + 8)) * 2 bytes.
 template <int CB>
 struct WFrag {
   uint4 b[4][CB];  // W^T fragments of four 32-wide k steps
@@ -208,7 +210,7 @@ __device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ As, in
 
 template <int CB>
 __global__ __launch_bounds__(256) void mlp_hidden_kernel(
-    const float* __restrict__ X, int64_t n, int d0, const float* __restrict__ mean,
+    const float* __restrict__ X, int64_t ldx, int64_t n, int d0, const float* __restrict__ mean,
     const float* __restrict__ inv, const uint16_t* __restrict__ W1T,
     const float* __restrict__ b1, int d1, const uint16_t* __restrict__ W2T,
     const float* __restrict__ b2, int d2, uint16_t* __restrict__ H2) {
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(256) void mlp_hidden_kernel(
   {  // normalised bf16 input tile: 8 threads per row, float4 each
     const int row = t >> 3;
     const bool in = m0 + row < n;
-    const float* xr = X + (m0 + row) * d0;
+    const float* xr = X + (m0 + row) * ldx;
     for (int c4 = (t & 7) * 4; c4 < d0; c4 += 32) {
       uint2 o = make_uint2(0, 0);
       if (in) {
@@ -632,6 +634,9 @@ void mlp_destroy(Mlp* m) { delete m; }
 rte_ctx::~rte_ctx() {
   rte::mlp_destroy(mlp);
   if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (side) (void)hipStreamDestroy(side);
 }
 
 namespace {
@@ -939,12 +944,16 @@ int rte_mlp_train_step(rte_ctx* c, const float* X, const int32_t* y, int n, floa
   return RTE_OK;
 }
 
-int rte_mlp_predict(rte_ctx* c, const float* X, int64_t n, const int32_t* fallback,
-                    float min_conf, int32_t* leaf_out, float* conf_out) {
-  if (!c) return RTE_EINVAL;
-  if (!c->mlp) return rte_fail(c, RTE_ESTATE, "rte_mlp_create not called");
-  if (!X || !leaf_out || n < 0) return rte_fail(c, RTE_EINVAL, "bad predict arguments");
-  if (n == 0) return RTE_OK;
+}  // extern "C"
+
+namespace {
+
+// Prediction of n rows of X (row stride ldx floats).  The fallback leaves
+// are `fallback`, or with sel_pq the kd-tree leaves of sel_pq (row stride
+// sel_stride), selected on the side stream while the network runs.
+int predict_impl(rte_ctx* c, const float* X, int64_t ldx, int64_t n, const int32_t* fallback,
+                 const float* sel_pq, int sel_stride, float min_conf, int32_t* leaf_out,
+                 float* conf_out) {
   Mlp* m = c->mlp;
   MCHK(c, hipSetDevice(c->device), "hipSetDevice");
   const int d0 = m->dims[0], d1 = m->dims[1], d2 = m->dims[2], d3 = m->dims[3];
@@ -953,8 +962,16 @@ int rte_mlp_predict(rte_ctx* c, const float* X, int64_t n, const int32_t* fallba
   // aligned input rows; other shapes take the training kernels
   const bool fast = d0 % 32 == 0 && d1 % 32 == 0 && d1 <= 512 &&
                     (d2 == 32 || d2 == 64 || d2 == 128 || d2 == 256 || d2 == 512) &&
-                    ((uintptr_t)X & 15) == 0;
+                    ((uintptr_t)X & 15) == 0 && ldx % 4 == 0;
+  if (sel_pq) MCHK(c, c->kd_hits.reserve((size_t)n * 4), "kd-tree leaves");
   if (!fast) {
+    if (ldx != d0) return rte_fail(c, RTE_EUNSUP, "strided input needs the fused kernels");
+    if (sel_pq) {
+      MCHK(c, rte::launch_select_entry(c->nodes.as<rte::BoxNode>(), c->n_nodes, sel_pq, n,
+                                       sel_stride, c->kd_hits.as<int32_t>(), c->stream),
+           "select entry");
+      fallback = c->kd_hits.as<int32_t>();
+    }
     const int chunk = 16384;
     MCHK(c, reserve_batch(m, (int)std::min<int64_t>(n, chunk)), "batch buffers");
     for (int64_t s = 0; s < n; s += chunk) {
@@ -1015,9 +1032,24 @@ int rte_mlp_predict(rte_ctx* c, const float* X, int64_t n, const int32_t* fallba
        "head kernel LDS size");
   MCHK(c, hipFuncSetAttribute(hk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_h),
        "hidden kernel LDS size");
+  if (sel_pq) {  // kd-tree leaves on the side stream, overlapping the network
+    if (!c->side) {
+      MCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking), "side stream");
+      MCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "event");
+      MCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "event");
+    }
+    MCHK(c, hipEventRecord(c->ev_fork, c->stream), "fork");
+    MCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0), "fork");
+    MCHK(c, rte::launch_select_entry(c->nodes.as<rte::BoxNode>(), c->n_nodes, sel_pq, n,
+                                     sel_stride, c->kd_hits.as<int32_t>(), c->side),
+         "select entry");
+    MCHK(c, hipEventRecord(c->ev_join, c->side), "join");
+    fallback = c->kd_hits.as<int32_t>();
+  }
+  bool joined = sel_pq == nullptr;
   for (int64_t s = 0; s < n; s += chunk) {
     const int64_t b = std::min<int64_t>(chunk, n - s);
-    const float* Xs = X + s * d0;
+    const float* Xs = X + s * ldx;
     const float* mean = m->in_mean.as<float>();
     const float* inv = m->in_inv.as<float>();
     const uint16_t* W1T = m->WT16[0].as<uint16_t>();
@@ -1027,8 +1059,8 @@ int rte_mlp_predict(rte_ctx* c, const float* X, int64_t n, const int32_t* fallba
     rte::SoftPart* parts = m->parts.as<rte::SoftPart>();
     const dim3 gh((unsigned)((b + 31) / 32));
 #define HID(CB)                                                                                   \
-  hipLaunchKernelGGL(rte::mlp_hidden_kernel<CB>, gh, dim3(256), lds_h, c->stream, Xs, b, d0, mean, \
-                     inv, W1T, m->b[0].as<float>(), d1, W2T, m->b[1].as<float>(), d2, H2)
+  hipLaunchKernelGGL(rte::mlp_hidden_kernel<CB>, gh, dim3(256), lds_h, c->stream, Xs, ldx, b, d0, \
+                     mean, inv, W1T, m->b[0].as<float>(), d1, W2T, m->b[1].as<float>(), d2, H2)
     switch (cb) {
       case 1: HID(1); break;
       case 2: HID(2); break;
@@ -1048,12 +1080,41 @@ int rte_mlp_predict(rte_ctx* c, const float* X, int64_t n, const int32_t* fallba
       default: HEAD(16); break;
     }
 #undef HEAD
+    if (!joined) {
+      MCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0), "join");
+      joined = true;
+    }
     hipLaunchKernelGGL(rte::predict_combine, dim3((unsigned)((b + 255) / 256)), dim3(256), 0,
                        c->stream, parts, G, b, fallback ? fallback + s : nullptr, min_conf,
                        leaf_out + s, conf_out ? conf_out + s : nullptr);
     MCHK(c, hipGetLastError(), "predict");
   }
   return RTE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rte_mlp_predict(rte_ctx* c, const float* X, int64_t n, const int32_t* fallback,
+                    float min_conf, int32_t* leaf_out, float* conf_out) {
+  if (!c) return RTE_EINVAL;
+  if (!c->mlp) return rte_fail(c, RTE_ESTATE, "rte_mlp_create not called");
+  if (!X || !leaf_out || n < 0) return rte_fail(c, RTE_EINVAL, "bad predict arguments");
+  if (n == 0) return RTE_OK;
+  return predict_impl(c, X, c->mlp->dims[0], n, fallback, nullptr, 0, min_conf, leaf_out,
+                      conf_out);
+}
+
+int rte_predict_entry(rte_ctx* c, const float* PQ, int64_t nq, int stride, float min_conf,
+                      int32_t* leaf_out, float* conf_out) {
+  if (!c) return RTE_EINVAL;
+  if (!c->mlp) return rte_fail(c, RTE_ESTATE, "rte_mlp_create not called");
+  if (!c->n_nodes) return rte_fail(c, RTE_ESTATE, "rte_set_leaves not called");
+  if (!PQ || !leaf_out || nq < 0 || stride < std::max(3, c->mlp->dims[0]))
+    return rte_fail(c, RTE_EINVAL, "bad predict_entry arguments (stride %d)", stride);
+  if (nq == 0) return RTE_OK;
+  return predict_impl(c, PQ, stride, nq, nullptr, PQ, stride, min_conf, leaf_out, conf_out);
 }
 
 }  // extern "C"

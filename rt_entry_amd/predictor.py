@@ -48,12 +48,13 @@ def candidate_leaves(kd_hits: np.ndarray, gt: np.ndarray, belong: np.ndarray, k:
 
 def make_labels(ctx: Context, PQ, cand, gt, beam: int = 64, width: int = 1,
                 budgets=(4, 8, 16), mode: int = 0, kd=None, n_leaf: int | None = None,
-                rows_per_call: int = 1 << 22):
+                rows_per_call: int = 1 << 22, top: int = 1):
     """Label per query (device search, device scoring).
 
     cand: (nq, C) int32 shortlist, -1 padded, column 0 the kd-tree leaf; or
     None to score all n_leaf leaves (kd = the kd-tree leaves for tie-breaks).
-    Returns (labels int32[nq], best score int32[nq])."""
+    Returns (labels int32[nq], best score int32[nq]); with top > 1 (all-leaf
+    scoring only) labels is (nq, top), the best `top` leaves in key order."""
     import torch
     dev = PQ.device
     nq = PQ.shape[0]
@@ -62,7 +63,7 @@ def make_labels(ctx: Context, PQ, cand, gt, beam: int = 64, width: int = 1,
     if cand is None:
         L = int(n_leaf)
         kd_t = torch.as_tensor(np.ascontiguousarray(kd), device=dev).long()
-        labels = np.empty(nq, np.int32)
+        labels = np.empty((nq, top) if top > 1 else nq, np.int32)
         best = np.empty(nq, np.int32)
         qc = max(1, rows_per_call // L)
         leaf_ids = torch.arange(L, device=dev, dtype=torch.int32)
@@ -81,7 +82,10 @@ def make_labels(ctx: Context, PQ, cand, gt, beam: int = 64, width: int = 1,
             key = hits * (2 * L) + (L - 1 - leaf_ids.long())[None, :]
             key[torch.arange(qn, device=dev), kd_t[q0:q0 + qn]] += L
             arg = key.argmax(1)
-            labels[q0:q0 + qn] = arg.int().cpu().numpy()
+            if top > 1:
+                labels[q0:q0 + qn] = key.topk(top, dim=1).indices.int().cpu().numpy()
+            else:
+                labels[q0:q0 + qn] = arg.int().cpu().numpy()
             best[q0:q0 + qn] = hits.gather(1, arg[:, None])[:, 0].int().cpu().numpy()
             del X, g, hits, key
         return labels, best
@@ -190,3 +194,30 @@ def fit_temperature(ctx: Context, validate, factors=(1.0, 1.5, 2.0, 3.0, 4.0, 6.
     b[2] *= np.float32(best[0])
     ctx.mlp_set(W, b)
     return best[0], hist
+
+
+def save(ctx: Context, path: str, **extra):
+    """Write the trained network (weights, biases, input normalisation) and
+    `extra` scalars to an .npz file."""
+    W, b = ctx.mlp_get()
+    d = {f"W{i}": W[i] for i in range(3)}
+    d.update({f"b{i}": b[i] for i in range(3)})
+    norm = getattr(ctx, "mlp_norm", None)
+    if norm is not None:
+        d["norm_mean"], d["norm_inv"] = norm
+    d.update({k: np.asarray(v) for k, v in extra.items()})
+    np.savez(path, **d)
+
+
+def load(ctx: Context, path: str) -> dict:
+    """Create the network on `ctx` from a file written by save(); returns
+    the extra scalars."""
+    z = np.load(path)
+    W = [z[f"W{i}"] for i in range(3)]
+    b = [z[f"b{i}"] for i in range(3)]
+    ctx.mlp_create(W[0].shape[0], W[0].shape[1], W[1].shape[1], W[2].shape[1])
+    ctx.mlp_set(W, b)
+    if "norm_mean" in z:
+        ctx.mlp_set_input_norm(z["norm_mean"], z["norm_inv"])
+    skip = {f"W{i}" for i in range(3)} | {f"b{i}" for i in range(3)} | {"norm_mean", "norm_inv"}
+    return {k: z[k].item() for k in z.files if k not in skip}

@@ -62,3 +62,45 @@ def test_predictor_pipeline_and_gain(ctx, int_workload):
     print(f"[pred] recall kd/learned: {rec}", flush=True)
     # the learned entries converge faster at a small hop budget
     assert rec[("learned", 4)] > rec[("kd", 4)]
+
+
+@pytest.mark.parametrize("hidden", [(64, 64), (48, 40)])
+def test_predict_entry_equals_select_then_predict(ctx, int_workload, hidden):
+    """rte_predict_entry (kd-tree selection on a second stream, overlapping
+    the network) gives exactly the leaves and confidences of
+    rte_select_entry followed by rte_mlp_predict, for thresholds that keep
+    every prediction, some, and none; (64, 64) takes the fused kernels with a
+    strided input view, (48, 40) the generic ones."""
+    import torch
+    w = int_workload
+    dev = torch.device("cuda:0")
+    ctx.set_leaves(w.aabbs, w.pids)
+    n_leaf = w.pids.shape[0]
+    d = w.queries.shape[1]
+    ctx.mlp_create(d, hidden[0], hidden[1], n_leaf, seed=3)
+    rng = np.random.default_rng(2)
+    W, b = ctx.mlp_get()
+    ctx.mlp_set([x * 4 for x in W], [rng.normal(0, 0.5, size=x.shape).astype(np.float32) for x in b])
+    ctx.mlp_set_input_norm(w.queries.mean(0), 1.0 / (w.queries.std(0) + 1e-3))
+    pq = torch.from_numpy(w.queries).to(dev)
+    if hidden == (64, 64):  # rows 8 floats wider than the network input
+        wide = torch.zeros((pq.shape[0], d + 8), dtype=torch.float32, device=dev)
+        wide[:, :d] = pq
+        X = wide[:, :d]
+    else:
+        X = pq
+    for min_conf in (0.0, 0.5, 1.5):
+        kd = ctx.select_entry(pq)
+        conf0 = torch.empty(pq.shape[0], dtype=torch.float32, device=dev)
+        ref = ctx.mlp_predict(pq, fallback=kd, min_conf=min_conf, conf=conf0)
+        conf1 = torch.empty_like(conf0)
+        got = ctx.predict_entry(X, min_conf=min_conf, conf=conf1)
+        ctx.synchronize()
+        ref, got = ref.cpu().numpy(), got.cpu().numpy()
+        print(f"[predict_entry] hidden {hidden} min_conf {min_conf}: "
+              f"{(ref != kd.cpu().numpy()).mean():.3f} not kd", flush=True)
+        np.testing.assert_array_equal(got, ref)
+        np.testing.assert_array_equal(conf1.cpu().numpy().view(np.int32),
+                                      conf0.cpu().numpy().view(np.int32))
+        if min_conf > 1:
+            np.testing.assert_array_equal(got, kd.cpu().numpy())

@@ -250,3 +250,69 @@ def test_search_launch_shapes_same_results(ctx, small_workload, wpb):
     ref, _ = ctx.search(w.queries, w.hits, make_params(k=10, beam=64))
     ids, _ = ctx.search(w.queries, w.hits, make_params(k=10, beam=64, wg_waves=wpb, grid=7))
     assert_same(ids, ref, f"wg_waves={wpb}")
+
+
+def test_batched_large_batch_prep(ctx, int_workload):
+    """A batch of ~17k queries with mixed k and beam (every beam class, many
+    preparation workgroups): the workload's queries tiled must give the
+    oracle's results tiled, for the whole batch and a prefix of it."""
+    import torch
+    from rt_entry_amd import data as D
+    from rt_entry_amd import make_params
+    w = int_workload
+    _ctx_for(ctx, w)
+    dev = torch.device("cuda:0")
+    nq = w.queries.shape[0]
+    kq, bq = D.k_beam_assignment(3, nq)
+    po = O.make_params(k=10, beam=64, mode=0)
+    io, do = O.search_uneven(w.base, w.graph, w.queries, w.pids, w.hits, kq, bq, po)
+    reps = -(-16385 // nq) + 1
+    print(f"[large batch] {nq} queries x {reps}", flush=True)
+    PQ = torch.from_numpy(np.tile(w.queries, (reps, 1))).to(dev)
+    leaf = torch.from_numpy(np.tile(w.hits, reps)).to(dev)
+    kqd = torch.from_numpy(np.tile(kq, reps)).to(dev)
+    bqd = torch.from_numpy(np.tile(bq, reps)).to(dev)
+    for n in (PQ.shape[0], nq * (16384 // nq)):
+        ids, dists = ctx.search_batched(PQ[:n], leaf[:n], kqd[:n], bqd[:n], make_params(mode=0),
+                                        k_stride=io.shape[1])
+        ctx.synchronize()
+        ids = ids.cpu().numpy().reshape(-1, nq, io.shape[1])
+        dists = dists.cpu().numpy().reshape(-1, nq, io.shape[1])
+        for r in range(ids.shape[0]):
+            for q in range(nq):
+                k = kq[q]
+                if not np.array_equal(ids[r, q, :k], io[q, :k]) or not np.array_equal(
+                        dists[r, q, :k].view(np.int32), do[q, :k].view(np.int32)):
+                    raise AssertionError(f"n={n} copy {r} query {q}: {ids[r, q, :k]} vs {io[q, :k]}")
+
+
+def test_batched_uniform_equals_search(ctx, int_workload):
+    """A uniform batch (no per-query arrays) is the search kernel alone in a
+    captured graph, walking the queries grid-stride: ids and distances equal
+    rte_search's for every launch shape, over several replays, and with
+    output rows wider than k."""
+    import torch
+    from rt_entry_amd import make_params
+    w = int_workload
+    _ctx_for(ctx, w)
+    dev = torch.device("cuda:0")
+    PQ = torch.from_numpy(w.queries).to(dev)
+    leaf = torch.from_numpy(w.hits).to(dev)
+    for beam, k in ((64, 10), (128, 16)):
+        p = make_params(k=k, beam=beam, max_iter=50)
+        ref_i, ref_d = ctx.search(PQ, leaf, p)
+        ref_i, ref_d = ref_i.cpu().numpy(), ref_d.cpu().numpy()
+        for wg, grid in ((4, 0), (1, 0), (2, 3), (1, 1000)):
+            q = make_params(k=k, beam=beam, max_iter=50, wg_waves=wg, grid=grid)
+            for rep in range(3):
+                ids, dists = ctx.search_batched(PQ, leaf, None, None, q, k_stride=k + 3)
+                ctx.synchronize()
+                ctx.search(PQ, leaf, make_params(k=10, beam=32))  # eager work in between
+                ctx.synchronize()
+                what = f"beam {beam} k {k} wg {wg} grid {grid} replay {rep}"
+                assert_same(ids.cpu().numpy()[:, :k], ref_i, what)
+                assert_same(dists.cpu().numpy()[:, :k].view(np.int32), ref_d.view(np.int32), what)
+        best, ms = ctx.autotune_batched(PQ, leaf, None, None, p, k_stride=k)
+        ids, dists = ctx.search_batched(PQ, leaf, None, None, best, k_stride=k)
+        ctx.synchronize()
+        assert_same(ids.cpu().numpy(), ref_i, f"autotuned wg {best.wg_waves} grid {best.grid}")
