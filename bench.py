@@ -257,16 +257,17 @@ def main():
         kd_h = kd_all[n_val:].cpu().numpy()
         kd_v = kd_all[:n_val].contiguous()
         gt_t, gt_v = W["gt_train"][n_val:], W["gt_train"][:n_val]
-        # the three best-converging leaves of every training query (scored by
+        # the five best-converging leaves of every training query (scored by
         # the plain search from every leaf), each a training pair: softer
         # targets than the single best leaf, which is often a near tie
-        lab3, _ = P.make_labels(ctx, PQt, None, gt_t, beam=64, mode=args.mode,
-                                kd=kd_h, n_leaf=W["n_leaf"], top=3)
-        labels = lab3[:, 0]
+        n_lab = 5
+        labm, _ = P.make_labels(ctx, PQt, None, gt_t, beam=64, mode=args.mode,
+                                kd=kd_h, n_leaf=W["n_leaf"], top=n_lab)
+        labels = labm[:, 0]
         t_lab = time.time() - t0
         log(f"labels in {t_lab:.1f}s; {(labels != kd_h).mean():.3f} differ from the kd leaf")
-        X3 = PQt.repeat(3, 1).contiguous()
-        y3 = torch.from_numpy(np.ascontiguousarray(lab3.T.reshape(-1))).to(dev)
+        X3 = PQt.repeat(n_lab, 1).contiguous()
+        y3 = torch.from_numpy(np.ascontiguousarray(labm.T.reshape(-1))).to(dev)
         leaf_v = torch.empty(n_val, dtype=torch.int32, device=dev)
         pv = make_params(k=k, beam=128, max_iter=32, mode=args.mode)
         min_conf = [0.5 if args.min_conf is None else args.min_conf]
@@ -297,7 +298,7 @@ def main():
             min_conf[0] = max(mc_hist, key=lambda x: (x[1], x[0]))[0]
             log(f"fallback threshold {min_conf[0]} ({mc_hist})")
         args.min_conf = min_conf[0]
-        pred_info = {"train_queries": int(PQt.shape[0]), "labels_per_query": 3,
+        pred_info = {"train_queries": int(PQt.shape[0]), "labels_per_query": n_lab,
                      "validation_queries": n_val,
                      "epochs_run": args.epochs, "best_epoch": best_ep[0],
                      "validation_recall@10_beam128_it32": round(best_ep[1], 4),

@@ -136,8 +136,10 @@ int rte_search(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stride,
  * beam.  p->k is ignored.
  * With k_per_q = beam_per_q = NULL the batch is uniform (every query uses
  * p->k and p->beam, k_stride >= p->k): the captured graph is the search
- * kernel alone, its waves walking the queries grid-stride over p->grid
- * workgroups (0: the resident maximum); results equal rte_search's. */
+ * kernel alone over p->grid workgroups (0: the resident maximum), its waves
+ * taking queries from a work counter that the launch leaves at zero;
+ * results equal rte_search's.  Batched calls on one context must not
+ * overlap (they share the counter). */
 int rte_search_batched(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stride,
                        const int32_t* leaf, const int32_t* k_per_q,
                        const int32_t* beam_per_q, const rte_params* p, int k_stride,

@@ -637,7 +637,12 @@ static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
     HIPCHK(c, rte::launch_search(a, beam_max, grid, wpb, c->stream), "search launch");
   } else if (!uneven) {
     // uniform batch: the captured graph is the search kernel alone; waves
-    // walk the queries grid-stride (no preparation, no work counter)
+    // take queries from the self-clearing work counter (zeroed once here)
+    if (!c->b_counter.p) {
+      HIPCHK(c, c->b_counter.reserve(16 * sizeof(int64_t)), "hipMalloc(counter)");
+      HIPCHK(c, hipMemsetAsync(c->b_counter.p, 0, 16 * sizeof(int64_t), c->stream), "counter");
+    }
+    a.work_counter = c->b_counter.as<int64_t>();
     int grid = p->grid;
     if (grid == 0) {
       int occ = 0, cus = 0;
@@ -654,7 +659,10 @@ static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
     HIPCHK(c, c->b_k.reserve(nq * sizeof(int32_t)), "hipMalloc(k_per_q)");
     HIPCHK(c, c->b_beam.reserve(nq * sizeof(int32_t)), "hipMalloc(beam_per_q)");
     HIPCHK(c, c->b_order.reserve(nq * sizeof(int32_t)), "hipMalloc(order)");
-    HIPCHK(c, c->b_counter.reserve(16 * sizeof(int64_t)), "hipMalloc(counter)");
+    if (!c->b_counter.p) {
+      HIPCHK(c, c->b_counter.reserve(16 * sizeof(int64_t)), "hipMalloc(counter)");
+      HIPCHK(c, hipMemsetAsync(c->b_counter.p, 0, 16 * sizeof(int64_t), c->stream), "counter");
+    }
     const int32_t* kin;
     const int32_t* bin;
     if (int r = stage_in(c, c->b_kin, k_per_q, (size_t)nq, where, &kin)) return r;

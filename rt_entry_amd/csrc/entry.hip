@@ -28,11 +28,21 @@ __global__ __launch_bounds__(256) void select_entry_kernel(const BoxNode* __rest
                                                            int32_t* __restrict__ hits) {
   __shared__ BoxNode top[kTopNodes];
   const int n_top = min(n_nodes, kTopNodes);
-  for (int i = threadIdx.x; i < n_top; i += blockDim.x) {
-    const uint4* src = reinterpret_cast<const uint4*>(nodes + i);
-    uint4* dst = reinterpret_cast<uint4*>(top + i);
-    dst[0] = src[0];
-    dst[1] = src[1];
+  {  // all loads of the copy first (one memory round trip), then the stores
+    constexpr int kPer = (kTopNodes * 2 + 255) / 256;  // 16-byte chunks per thread
+    const uint4* src = reinterpret_cast<const uint4*>(nodes);
+    uint4* dst = reinterpret_cast<uint4*>(top);
+    uint4 v[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int i = (int)threadIdx.x + 256 * j;
+      v[j] = i < 2 * n_top ? src[i] : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int i = (int)threadIdx.x + 256 * j;
+      if (i < 2 * n_top) dst[i] = v[j];
+    }
   }
   __syncthreads();
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -10,8 +10,9 @@
 // Tiling: a workgroup computes a 64 x 64 output tile (four waves of 16 rows,
 // each four 16x16 MFMA accumulators).  K is processed in chunks of 128: the
 // chunk of R's 64 columns is staged into LDS transposed ([col][k], one
-// float4 per operand read) while each lane loads its Q fragments of the
-// chunk (one float4 per 16 k), so a chunk costs one memory round trip.
+// float4 per operand read); its loads and each lane's Q fragments of the
+// chunk (one float4 per 16 k) are all issued before any is used, so a chunk
+// costs one memory round trip.
 // Inside a 16-wide k step the MFMA's k index g maps to k0 + 4g + c for
 // component c of the lane's float4; that only permutes the k sum.
 #include <hip/hip_runtime.h>
@@ -46,16 +47,15 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
   for (int u = 0; u < 4; ++u) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
   for (int kc = 0; kc < d; kc += kProjChunk) {
     // R[kc .. kc+128)[col0 .. col0+64) -> Rt[col][k]: thread t handles column
-    // t % 64 and k = kc + t / 64 + 4 j
-    {
-      const int c = t & 63, k_first = t >> 6;
-      const int col = col0 + c;
-#pragma unroll 8
-      for (int j = 0; j < kProjChunk / 4; ++j) {
-        const int kl = k_first + 4 * j;
-        const int k = kc + kl;
-        Rt[c * kProjLds + kl] = (k < d && col < d_out) ? R[(int64_t)k * d_out + col] : 0.f;
-      }
+    // t % 64 and k = kc + t / 64 + 4 j; all R and Q loads of the chunk are
+    // issued before the first is used
+    const int rc = t & 63, k_first = t >> 6;
+    const int col = col0 + rc;
+    float rv[kProjChunk / 4];
+#pragma unroll
+    for (int j = 0; j < kProjChunk / 4; ++j) {
+      const int k = kc + k_first + 4 * j;
+      rv[j] = (k < d && col < d_out) ? R[(int64_t)k * d_out + col] : 0.f;
     }
     float4 qa[kProjChunk / 16];
 #pragma unroll
@@ -70,6 +70,8 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
         qa[s].w = (arow_ok && ka + 3 < d) ? Q[arow * d + ka + 3] : 0.f;
       }
     }
+#pragma unroll
+    for (int j = 0; j < kProjChunk / 4; ++j) Rt[rc * kProjLds + k_first + 4 * j] = rv[j];
     __syncthreads();
     const int steps = min(kProjChunk, d - kc + 15) / 16;
 #pragma unroll

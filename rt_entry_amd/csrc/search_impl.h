@@ -469,7 +469,8 @@ __device__ __forceinline__ uint32_t scan_chunk(const uint64_t (&bf)[NR], const u
 // Next query of this wave.  Without a work counter: a grid stride.  With one
 // (persistent launch): the first query of every wave is its global wave id
 // (no atomic), later ones come from the counter, which counts past the
-// statically assigned prefix; either way through the order array.
+// statically assigned prefix and is zero again when the launch ends; either
+// way through the order array when there is one.
 __device__ __forceinline__ int64_t next_query(const SearchArgs& a, int64_t prev_slot, int wave,
                                               int lane) {
   const int wpb = blockDim.x >> 6;
@@ -482,8 +483,15 @@ __device__ __forceinline__ int64_t next_query(const SearchArgs& a, int64_t prev_
   } else if (waves >= a.nq) {
     slot = a.nq;  // every query was assigned statically
   } else {
+    unsigned long long* ctr = reinterpret_cast<unsigned long long*>(a.work_counter);
     unsigned long long i = 0;
-    if (lane == 0) i = atomicAdd(reinterpret_cast<unsigned long long*>(a.work_counter), 1ull);
+    if (lane == 0) {
+      i = atomicAdd(ctr, 1ull);
+      // a launch makes nq - waves increments that get a query and one per
+      // wave that does not: nq in all, so the one returning nq - 1 is the
+      // last, and it clears the counter for the next launch
+      if (i == (unsigned long long)(a.nq - 1)) atomicExch(ctr, 0ull);
+    }
     slot = waves + (int64_t)__builtin_amdgcn_readfirstlane((int)i);
   }
   return slot;

@@ -80,6 +80,15 @@ def int_workload():
 
 
 @pytest.fixture(scope="session")
+def wide_workload():
+    """960-d low-rank mixture (workload 4's distribution), degree-48 graph
+    stored at stride 64: the 30-dims-per-lane kernel."""
+    if "wide" not in _cache:
+        _cache["wide"] = Workload(4, n=3000, nq=48, degree=48, k_fwd=64, rev=48, cap=64)
+    return _cache["wide"]
+
+
+@pytest.fixture(scope="session")
 def deg48_workload():
     """Degree-48 graph (row stride 64), 96-d unit vectors."""
     if "deg48" not in _cache:

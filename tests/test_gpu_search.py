@@ -316,3 +316,18 @@ def test_batched_uniform_equals_search(ctx, int_workload):
         ids, dists = ctx.search_batched(PQ, leaf, None, None, best, k_stride=k)
         ctx.synchronize()
         assert_same(ids.cpu().numpy(), ref_i, f"autotuned wg {best.wg_waves} grid {best.grid}")
+
+
+@pytest.mark.parametrize("mode,beam,width", [(0, 64, 1), (1, 128, 1), (0, 64, 2)])
+def test_search_parity_960d(ctx, wide_workload, mode, beam, width):
+    """960-d vectors (30 dims per lane, the reference's largest DIM) with a
+    degree-48 graph: ids, distances and stats equal the oracle's."""
+    w = wide_workload
+    _ctx_for(ctx, w)
+    print(f"[parity] 960-d mode={mode} beam={beam} width={width}", flush=True)
+    (io, do, so), (ig, dg, sg) = _run_both(ctx, w, w.queries, w.hits, k=10, beam=beam,
+                                           mode=mode, width=width, max_iter=40)
+    assert_same(ig, io, "ids")
+    assert_same(dg.view(np.int32), do.view(np.int32), "dists")
+    assert_same(sg, so.astype(np.int32), "stats")
+    assert O.recall(ig, w.gt) > 0.5
