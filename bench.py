@@ -37,7 +37,7 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def prepare(workload: int, scale: float, dev, n_train: int = 0):
+def prepare(workload: int, scale: float, dev, n_train: int = 0, keep_raw: bool = False):
     import torch
 
     import oracle as O
@@ -81,6 +81,8 @@ def prepare(workload: int, scale: float, dev, n_train: int = 0):
         out["train_raw"] = tr
         out["gt_train"] = G.knn_gpu(base, tr, 10)[0].cpu().numpy()
         log(f"{n_train} training queries + exact 10-NN in {time.time() - t0:.1f}s")
+    if keep_raw:  # original vectors for the full-precision rerank
+        out["raw"], out["raw_h"], out["Q_h"] = base, base_h, q_h
     del base
     return out
 
@@ -152,6 +154,9 @@ def main():
                     help="measure only one entry method (profiling runs)")
     ap.add_argument("--save-predictor", default="", help="write the trained network (.npz)")
     ap.add_argument("--load-predictor", default="", help="use a saved network, no training")
+    ap.add_argument("--search-dims", type=int, default=0,
+                    help="search on the first D PCA dimensions, then rerank the whole beam "
+                         "in the original space (the reference's DIM < d with REORDER)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after timing, 5 device-to-device copies of this many MiB (a "
                          "known-size kernel for scaling the HBM byte counters)")
@@ -179,7 +184,7 @@ def main():
     from rt_entry_amd import predictor as P
 
     dev = torch.device("cuda:0")
-    W = prepare(args.workload, args.scale, dev, args.n_train)
+    W = prepare(args.workload, args.scale, dev, args.n_train, keep_raw=args.search_dims > 0)
     nq = W["Q"].shape[0]
     k = 10
     ctx = Context(0)
@@ -194,6 +199,11 @@ def main():
     ctx.set_graph(W["graph"])
     ctx.set_leaves(W["aabbs"], W["pids"])
     ctx.set_pca(W["R"], W["mu"], W["d"])
+    sd = args.search_dims if 0 < args.search_dims < W["d"] else 0
+    d_search = sd or W["d"]
+    if sd:  # the graph walk reads the first sd PCA columns (row stride d)
+        ctx.set_base(W["base"], d=sd)
+        ctx.set_rerank_base(W["raw"])
 
     pq = torch.empty((nq, W["d"]), dtype=torch.float32, device=dev)
     hits = torch.empty(nq, dtype=torch.int32, device=dev)
@@ -210,9 +220,36 @@ def main():
             return ctx.predict_entry(pq, min_conf=args.min_conf, out=learned)
         return ctx.select_entry(pq, out=hits)
 
+    cand = {}
+
+    def with_k(p, kk):
+        return make_params(k=kk, beam=p.beam, width=p.width, max_iter=p.max_iter, mode=p.mode,
+                           algo=p.algo, bloom_words=p.bloom_words, bloom_hashes=p.bloom_hashes,
+                           wg_waves=p.wg_waves, grid=p.grid)
+
+    def cand_bufs(beam):
+        """Whole-beam outputs of the truncated search (rerank candidates)."""
+        if beam not in cand:
+            cand[beam] = (torch.empty((nq, beam), dtype=torch.int32, device=dev),
+                          torch.empty((nq, beam), dtype=torch.float32, device=dev))
+        return cand[beam]
+
+    def rerank(p):
+        ctx.rerank(W["Q"], cand_bufs(p.beam)[0], k, out=(ids, dists))
+
+    def search(entry, p, stats=None):
+        """The search stage: k results, or with truncated dimensions the whole
+        beam (reranked by the caller)."""
+        if sd:
+            ctx.search(pq, entry, with_k(p, p.beam), out=(*cand_bufs(p.beam), stats))
+        else:
+            ctx.search(pq, entry, p, out=(ids, dists, stats))
+
     def step(p):
         ctx.project(W["Q"], out=pq)
-        ctx.search(pq, entries(use_mlp[0]), p, out=(ids, dists, None))
+        search(entries(use_mlp[0]), p)
+        if sd:
+            rerank(p)
 
     def recall_of(p):
         step(p)
@@ -332,18 +369,22 @@ def main():
             # holds the search kernel alone)
             ctx.project(W["Q"], out=pq)
             entry0 = entries(use)  # entries for the tuning batch
-            tuned, tms = ctx.autotune_batched(pq, entry0, None, None, p, k_stride=k)
+            pt = with_k(p, best[1]) if sd else p
+            tuned, tms = ctx.autotune_batched(pq, entry0, None, None, pt, k_stride=pt.k)
             log(f"autotune: wg_waves={tuned.wg_waves} grid={tuned.grid} {tms:.3f} ms")
 
         def run_search(entry, stats=None):
             if batched:
-                ctx.search_batched(pq, entry, None, None, tuned, k_stride=k, out=(ids, dists))
+                ctx.search_batched(pq, entry, None, None, tuned, k_stride=tuned.k,
+                                   out=cand_bufs(best[1]) if sd else (ids, dists))
             else:
-                ctx.search(pq, entry, p, out=(ids, dists, stats))
+                search(entry, p, stats)
 
         for _ in range(args.warmup):
             ctx.project(W["Q"], out=pq)
             run_search(entries(use))
+            if sd:
+                rerank(p)
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
               for _ in range(args.steps)]
@@ -354,6 +395,8 @@ def main():
             ev[i][0].record(stream)
             run_search(entry)
             ev[i][1].record(stream)
+            if sd:
+                rerank(p)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         rec = O.recall(ids.cpu().numpy(), W["gt"], 10)
@@ -368,11 +411,13 @@ def main():
             torch.cuda.synchronize()
             pred_ms = e0.elapsed_time(e1) / args.steps
         stats = torch.empty((nq, 3), dtype=torch.int32, device=dev)
-        ctx.search(pq, entry, p, out=(ids, dists, stats))
+        search(entry, p, stats)
         torch.cuda.synchronize()
         st = stats.cpu().numpy().astype(np.int64)
         D = 1 << (W["cfg"]["degree"] - 1).bit_length()
-        alg = float((4 * W["d"] * (st[:, 0] + 1) + 4 * D * st[:, 1] + 4 * best[1] + 8 * k).sum())
+        k_out = best[1] if sd else k
+        alg = float((4 * d_search * (st[:, 0] + 1) + 4 * D * st[:, 1] + 4 * best[1]
+                     + 8 * k_out).sum())
         return dict(qps=nq * args.steps / wall, ms=wall / args.steps * 1e3, recall=rec,
                     search_ms=search_ms, alg_bytes=alg, p=p, entry=entry, beam=best[1],
                     max_iter=best[2], iters=float(st[:, 2].mean()), pred_ms=pred_ms,
@@ -412,7 +457,7 @@ def main():
         for tj in json.load(open(tf)):
             if (tj.get("beam"), tj.get("max_iter"), tj.get("entry")) == (
                     head["beam"], head["max_iter"], entry) and args.workload == 2 and \
-                    args.scale == 1.0:
+                    args.scale == 1.0 and not sd:
                 traffic = tj.get("hbm_bytes_per_launch")
     achieved = head["alg_bytes"] / (head["search_ms"] * 1e-3)
 
@@ -428,13 +473,22 @@ def main():
         g_h = W["graph"].cpu().numpy()
         from rt_entry_amd.data import pad_graph
         g_h = pad_graph(g_h, W["n"])
-        po = O.make_params(k=k, beam=head["beam"], max_iter=head["max_iter"], mode=args.mode)
+        po = O.make_params(k=head["beam"] if sd else k, beam=head["beam"],
+                           max_iter=head["max_iter"], mode=args.mode)
         threads = min(os.cpu_count() or 1, 16)
+        base_s = np.ascontiguousarray(W["base_h"][:, :sd]) if sd else W["base_h"]
+        pq_s = np.ascontiguousarray(pq_h[:, :sd]) if sd else pq_h
+
+        def cpu_search(m):
+            ci, _ = O.search(base_s, g_h, pq_s[:m], W["pids"], ent_h[:m], po, n_threads=threads)
+            if sd:
+                O.rerank(W["raw_h"], W["Q_h"][:m], ci, k, n_threads=threads)
+
         m = 256
         t = 0.0
         while True:
             t0 = time.perf_counter()
-            O.search(W["base_h"], g_h, pq_h[:m], W["pids"], ent_h[:m], po, n_threads=threads)
+            cpu_search(m)
             t = time.perf_counter() - t0
             if t >= args.cpu_seconds / 4 or m >= nq:
                 break
@@ -443,13 +497,13 @@ def main():
         times = []
         for _ in range(reps):
             t0 = time.perf_counter()
-            O.search(W["base_h"], g_h, pq_h[:m], W["pids"], ent_h[:m], po, n_threads=threads)
+            cpu_search(m)
             times.append(time.perf_counter() - t0)
         cpu = {"value": round(m / float(np.median(times)), 1), "unit": "queries/s",
                "cores": threads, "kind": "port",
                "sample": f"first {m} of the {nq} queries with the same PCA queries, entry "
                          f"leaves, graph and parameters as the headline run; search stage "
-                         f"only; median of {reps} runs"}
+                         f"only{' (+ rerank)' if sd else ''}; median of {reps} runs"}
 
     def summary(r):
         if r is None:
@@ -480,6 +534,7 @@ def main():
             "entry": ("learned (kd-tree fallback), batched launch" if head is ln
                       else "kd-tree"),
             "width": 1, "mode": "tuned" if args.mode else "ref",
+            "search_dims": d_search, "rerank": bool(sd),
             "kd_tree_entry": summary(kd),
             "kd_tree_entry_batched": summary(kd_b),
             "learned_entry_batched": summary(ln),

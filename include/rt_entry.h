@@ -126,6 +126,19 @@ int rte_select_entry(rte_ctx* ctx, const float* PQ, int64_t nq, int stride,
 int rte_search(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stride,
                const int32_t* leaf, const rte_params* p, int32_t* ids_out,
                float* dists_out, int32_t* stats_out, int where);
+/* Full-precision rerank (REORDER): for each query, the distances of its
+ * n_cand candidate ids (e.g. the whole beam of rte_search with k = beam on
+ * truncated PCA dimensions) are recomputed on the rerank base in the
+ * original space, eight partial sums over dims l mod 8 added in order as the
+ * reference's AVX loop does, and the k best by (dist, id) are written
+ * (nq x k).  Ids outside the rerank base get +inf.  1 <= k <= n_cand <= 512,
+ * d % 8 == 0, strides multiples of 4 floats.  Replaces the host
+ * Graph::parallel_reorder (graph_helper.cu:146-213) called from
+ * Graph::GraphSearch under REORDER (graph.cu:326-336). */
+int rte_set_rerank_base(rte_ctx* ctx, const float* base, int64_t np, int d, int64_t stride,
+                        int where);
+int rte_rerank(rte_ctx* ctx, const float* Q, int64_t nq, int64_t q_stride, const int32_t* cand,
+               int n_cand, int k, int32_t* ids_out, float* dists_out, int where);
 /* Batched uneven search: per-query k and beam, one persistent launch with a
  * device work queue (longest beams first), captured as a hipGraph on first
  * use and replayed while the launch shape is unchanged; no host sync.  Per

@@ -71,6 +71,10 @@ def lib():
         L.orc_search_uneven.restype = C.c_int
         L.orc_sort_desc.argtypes = [P, P, C.c_int, C.c_int]
         L.orc_merge_asc.argtypes = [P, P, C.c_int, C.c_int]
+        L.orc_rerank_distance.argtypes = [P, P, C.c_int]
+        L.orc_rerank_distance.restype = C.c_float
+        L.orc_rerank.argtypes = [P, i64, i64, C.c_int, P, i64, i64, P, C.c_int, C.c_int, P, P,
+                                 C.c_int]
         L.orc_recall.argtypes = [P, i64, C.c_int, P, C.c_int, C.c_int]
         L.orc_recall.restype = C.c_double
         L.orc_num_threads.restype = C.c_int
@@ -238,6 +242,30 @@ def search_uneven(base, graph, queries, pids, hits, k_per_q, beam_per_q, params:
                                  _p(dists), int(n_threads))
     if rc != 0:
         raise RuntimeError(f"orc_search_uneven failed rc={rc}")
+    return ids, dists
+
+
+def rerank_distance(p, q) -> float:
+    """graph_helper.cu:161-190 distance of two rows (d % 8 == 0)."""
+    a = _c(p, np.float32)
+    b = _c(q, np.float32)
+    return float(lib().orc_rerank_distance(_p(a), _p(b), a.shape[0]))
+
+
+def rerank(base, queries, cand, k: int, n_threads: int = 0):
+    """Full-precision rerank of candidate ids (graph_helper.cu:146-213):
+    (ids, dists) of the k best candidates by (dist, id) per query."""
+    b = _c(base, np.float32)
+    qv = _c(queries, np.float32)
+    c = _c(cand, np.int32)
+    nq, n_cand = c.shape
+    ids = np.empty((nq, k), np.int32)
+    dists = np.empty((nq, k), np.float32)
+    rc = lib().orc_rerank(_p(b), b.shape[0], b.shape[1], b.shape[1], _p(qv), nq, qv.shape[1],
+                          _p(c), n_cand, int(k), _p(ids), _p(dists),
+                          int(n_threads) or num_threads())
+    if rc != 0:
+        raise ValueError("rerank arguments (d % 8 == 0, 1 <= k <= candidates)")
     return ids, dists
 
 

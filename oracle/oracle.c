@@ -585,6 +585,63 @@ int orc_search_uneven(const float* base, int64_t np, int64_t stride, int d,
   return 0;
 }
 
+/* graph_helper.cu:161-190: eight accumulators over dims l, l + 8, ...
+ * (mul then add), then accumulators 0..7 summed in order. */
+float orc_rerank_distance(const float* p, const float* q, int d) {
+  float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int j = 0; j < d; j += 8)
+    for (int l = 0; l < 8; ++l) {
+      const float diff = p[j + l] - q[j + l];
+      const float sq = diff * diff;
+      acc[l] = acc[l] + sq;
+    }
+  float dis = 0.0f;
+  for (int l = 0; l < 8; ++l) dis = dis + acc[l];
+  return dis;
+}
+
+typedef struct {
+  float d;
+  int32_t id;
+} orc_pair;
+
+static int pair_less(const orc_pair* a, const orc_pair* b) {
+  return a->d < b->d || (a->d == b->d && a->id < b->id);
+}
+
+int orc_rerank(const float* base, int64_t np, int64_t stride, int d, const float* queries,
+               int64_t nq, int64_t q_stride, const int32_t* cand, int n_cand, int k,
+               int32_t* ids_out, float* dists_out, int n_threads) {
+  if (d % 8 != 0 || d <= 0 || k < 1 || k > n_cand || stride < d || q_stride < d) return -1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads > 0 ? n_threads : 1)
+  for (int64_t q = 0; q < nq; ++q) {
+    orc_pair* v = (orc_pair*)malloc(sizeof(orc_pair) * (size_t)n_cand);
+    const float* qv = queries + q * q_stride;
+    for (int j = 0; j < n_cand; ++j) {
+      const int32_t id = cand[q * n_cand + j];
+      v[j].id = id;
+      v[j].d = (id >= 0 && id < np) ? orc_rerank_distance(base + (int64_t)id * stride, qv, d)
+                                    : INFINITY;
+    }
+    /* insertion sort by (dist, id): n_cand <= 512 */
+    for (int i = 1; i < n_cand; ++i) {
+      orc_pair x = v[i];
+      int j = i - 1;
+      while (j >= 0 && pair_less(&x, &v[j])) {
+        v[j + 1] = v[j];
+        --j;
+      }
+      v[j + 1] = x;
+    }
+    for (int j = 0; j < k; ++j) {
+      ids_out[q * k + j] = v[j].id;
+      if (dists_out) dists_out[q * k + j] = v[j].d;
+    }
+    free(v);
+  }
+  return 0;
+}
+
 /* graph_helper.cu:64-121 */
 double orc_recall(const int32_t* ids, int64_t nq, int k, const int32_t* gt, int gt_k,
                   int at) {

@@ -106,6 +106,19 @@ int orc_search_uneven(const float* base, int64_t np, int64_t stride, int d,
 void orc_sort_desc(float* d, int32_t* id, int m, int mode);
 void orc_merge_asc(float* d, int32_t* id, int n, int mode);
 
+/* Full-precision rerank (graph_helper.cu:146-213 Graph::parallel_reorder,
+ * enabled by REORDER in graph.cu:326-336): distance of every candidate in
+ * the original space as eight partial sums (dims j = l mod 8, sequential in
+ * j, product then add, no fused multiply-add), the partials added in order
+ * 0..7; candidates sorted by (dist, id) ascending (the reference's
+ * std::sort leaves ties unspecified); the first k written.  Ids outside
+ * [0, np) get dist +inf (the reference reads past the base).  d % 8 == 0
+ * (the reference's remainder handling is broken otherwise). */
+float orc_rerank_distance(const float* p, const float* q, int d);
+int orc_rerank(const float* base, int64_t np, int64_t stride, int d, const float* queries,
+               int64_t nq, int64_t q_stride, const int32_t* cand, int n_cand, int k,
+               int32_t* ids_out, float* dists_out, int n_threads);
+
 /* graph_helper.cu:64-121: recall@at of k result ids against gt rows. */
 double orc_recall(const int32_t* ids, int64_t nq, int k, const int32_t* gt, int gt_k,
                   int at);

@@ -318,6 +318,37 @@ class Context:
                                         float(min_conf), C.c_void_p(out.data_ptr()), cp))
         return out
 
+    def set_rerank_base(self, base):
+        """Full-precision vectors for rte_rerank (np x d f32, host or device)."""
+        p, w, keep = _ptr(base, np.float32)
+        self._rr_keep = keep if w == RTE_DEVICE else None
+        self._chk(lib().rte_set_rerank_base(self._h, p, int(base.shape[0]), int(base.shape[1]),
+                                            int(base.shape[1]), w))
+
+    def rerank(self, Q, cand, k: int = 10, out=None):
+        """(ids, dists) of the k best candidates per query by full-precision
+        distance; Q and cand host arrays or device tensors."""
+        pq, w, keep = _ptr(Q, np.float32)
+        nq, qs = Q.shape
+        n_cand = cand.shape[1]
+        if out is None:
+            if w == RTE_DEVICE:
+                import torch
+                ids = torch.empty((nq, k), dtype=torch.int32, device=Q.device)
+                dists = torch.empty((nq, k), dtype=torch.float32, device=Q.device)
+            else:
+                ids = np.empty((nq, k), np.int32)
+                dists = np.empty((nq, k), np.float32)
+        else:
+            ids, dists = out
+        pc, wc, keep2 = _ptr(cand, np.int32)
+        if wc != w:
+            raise ValueError("cand must live with the queries")
+        pi, _, _ = _ptr(ids, np.int32)
+        pd, _, _ = _ptr(dists, np.float32)
+        self._chk(lib().rte_rerank(self._h, pq, nq, qs, pc, int(n_cand), int(k), pi, pd, w))
+        return ids, dists
+
     def predict_entry(self, PQ, min_conf: float = 0.5, out=None, conf=None):
         """Entry leaves of device PCA queries: the network's leaf, or the
         kd-tree leaf below min_conf (the kd-tree selection overlaps the

@@ -17,6 +17,7 @@
 #include "entry.h"
 #include "kdtree.h"
 #include "project.h"
+#include "rerank.h"
 #include "search.h"
 
 namespace {
@@ -235,6 +236,64 @@ int rte_set_base(rte_ctx* c, const float* base, int64_t np, int d, int64_t strid
   c->np = np;
   c->d = d;
   c->stride = stride;
+  return RTE_OK;
+}
+
+int rte_set_rerank_base(rte_ctx* c, const float* base, int64_t np, int d, int64_t stride,
+                        int where) {
+  if (!c) return RTE_EINVAL;
+  if (!base || np <= 0 || np >= 0x7fffffff || d <= 0 || d % 8 != 0 || stride < d ||
+      stride % 4 != 0)
+    return fail(c, RTE_EINVAL, "bad rerank base np=%lld d=%d stride=%lld (d %% 8, stride %% 4)",
+                (long long)np, d, (long long)stride);
+  if (int r = set_device(c)) return r;
+  if (where == RTE_HOST) {
+    const size_t bytes = (size_t)np * stride * sizeof(float);
+    HIPCHK(c, c->rr_own.reserve(bytes), "hipMalloc(rerank base)");
+    HIPCHK(c, hipMemcpyAsync(c->rr_own.p, base, bytes, hipMemcpyHostToDevice, c->stream),
+           "copy rerank base");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "sync rerank base");
+    c->rr_base = c->rr_own.as<float>();
+  } else {
+    if (((uintptr_t)base & 15) != 0) return fail(c, RTE_EINVAL, "rerank base not 16-byte aligned");
+    c->rr_own.release();
+    c->rr_base = base;
+  }
+  c->rr_np = np;
+  c->rr_d = d;
+  c->rr_stride = stride;
+  return RTE_OK;
+}
+
+int rte_rerank(rte_ctx* c, const float* Q, int64_t nq, int64_t q_stride, const int32_t* cand,
+               int n_cand, int k, int32_t* ids_out, float* dists_out, int where) {
+  if (!c) return RTE_EINVAL;
+  if (!c->rr_base) return fail(c, RTE_ESTATE, "rte_set_rerank_base not called");
+  if (!Q || !cand || !ids_out || nq < 0 || q_stride < c->rr_d || q_stride % 4 != 0)
+    return fail(c, RTE_EINVAL, "bad rerank args (q_stride %lld, d %d)", (long long)q_stride,
+                c->rr_d);
+  if (n_cand < 1 || n_cand > rte::kRerankMaxCand || k < 1 || k > n_cand)
+    return fail(c, RTE_EINVAL, "rerank needs 1 <= k <= n_cand <= %d (k %d, n_cand %d)",
+                rte::kRerankMaxCand, k, n_cand);
+  if (nq == 0) return RTE_OK;
+  if (int r = set_device(c)) return r;
+  const float* dq;
+  const int32_t* dc;
+  int32_t* di;
+  float* dd = nullptr;
+  if (int r = stage_in(c, c->s_in, Q, (size_t)nq * q_stride, where, &dq)) return r;
+  if (where == RTE_DEVICE && ((uintptr_t)Q & 15) != 0)
+    return fail(c, RTE_EINVAL, "queries not 16-byte aligned");
+  if (int r = stage_in(c, c->s_aux, cand, (size_t)nq * n_cand, where, &dc)) return r;
+  if (int r = stage_out(c, c->s_out, ids_out, (size_t)nq * k, where, &di)) return r;
+  if (dists_out)
+    if (int r = stage_out(c, c->s_aux2, dists_out, (size_t)nq * k, where, &dd)) return r;
+  HIPCHK(c, rte::launch_rerank(c->rr_base, c->rr_np, c->rr_stride, c->rr_d, dq, nq, q_stride, dc,
+                               n_cand, k, di, dd, c->stream),
+         "rerank launch");
+  if (int r = finish_out(c, ids_out, di, (size_t)nq * k, where)) return r;
+  if (dists_out)
+    if (int r = finish_out(c, dists_out, dd, (size_t)nq * k, where)) return r;
   return RTE_OK;
 }
 

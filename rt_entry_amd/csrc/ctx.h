@@ -65,6 +65,12 @@ struct rte_ctx {
   DevBuf bias;  // d_out: mu R
   int pca_d = 0, pca_out = 0;
 
+  // rerank: full-precision vectors (original space)
+  const float* rr_base = nullptr;
+  int64_t rr_np = 0, rr_stride = 0;
+  int rr_d = 0;
+  DevBuf rr_own;
+
   // leaves
   DevBuf nodes;
   int n_nodes = 0;

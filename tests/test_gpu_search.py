@@ -331,3 +331,39 @@ def test_search_parity_960d(ctx, wide_workload, mode, beam, width):
     assert_same(dg.view(np.int32), do.view(np.int32), "dists")
     assert_same(sg, so.astype(np.int32), "stats")
     assert O.recall(ig, w.gt) > 0.5
+
+
+@pytest.mark.parametrize("dim,beam,k", [(64, 64, 10), (32, 256, 10), (64, 128, 128)])
+def test_rerank_after_truncated_search(ctx, small_workload, dim, beam, k):
+    """REORDER path: search on the first `dim` PCA columns returning the whole
+    beam, then full-precision rerank on the original vectors: ids and
+    distances equal the oracle's (same candidates, same association), on
+    device and host buffers, with unfilled slots (id np) in the candidates."""
+    import torch
+    from rt_entry_amd import make_params
+    w = small_workload
+    ctx.set_base(w.base, d=dim)
+    ctx.set_graph(w.graph_raw)
+    ctx.set_leaves(w.aabbs, w.pids)
+    ctx.set_rerank_base(w.raw_base)
+    q = np.ascontiguousarray(w.queries[:, :dim])
+    cand, _ = ctx.search(q, w.hits, make_params(k=beam, beam=beam, max_iter=60))
+    io_c, _ = O.search(np.ascontiguousarray(w.base[:, :dim]), w.graph, q, w.pids, w.hits,
+                       O.make_params(k=beam, beam=beam, max_iter=60))
+    assert_same(cand, io_c, "beam ids")
+    cand = cand.copy()
+    cand[0, -3:] = w.n  # unfilled slots sort last
+    print(f"[rerank] dim={dim} beam={beam} k={k}", flush=True)
+    io, do = O.rerank(w.raw_base, w.raw_queries, cand, k)
+    ih, dh = ctx.rerank(w.raw_queries, cand, k)  # host buffers
+    assert_same(ih, io, "host ids")
+    assert_same(dh.view(np.int32), do.view(np.int32), "host dists")
+    dev = torch.device("cuda:0")
+    idd, ddd = ctx.rerank(torch.from_numpy(w.raw_queries).to(dev), torch.from_numpy(cand).to(dev), k)
+    ctx.synchronize()
+    assert_same(idd.cpu().numpy(), io, "device ids")
+    assert_same(ddd.cpu().numpy().view(np.int32), do.view(np.int32), "device dists")
+    if k == 10:  # the rerank recovers recall lost to the truncated search
+        print(f"[rerank] recall search-only {O.recall(cand[:, :10], w.gt):.4f} "
+              f"reranked {O.recall(io, w.gt):.4f}", flush=True)
+        assert O.recall(io, w.gt) >= O.recall(np.ascontiguousarray(cand[:, :10]), w.gt)

@@ -363,3 +363,43 @@ def test_search_rejects_bad_params(small_workload):
     w = small_workload
     with pytest.raises(RuntimeError):
         O.search(w.base, w.graph, w.queries, w.pids, w.hits, O.make_params(k=100, beam=64))
+
+
+# ---- full-precision rerank (graph_helper.cu:146-213) -------------------
+
+def test_rerank_distance_association():
+    """Eight partials over dims l mod 8, summed 0..7: one case whose value
+    separates that grouping from a left-to-right sum (2^24 + 14 vs 2^24)."""
+    p = np.zeros(16, np.float32)
+    q = np.zeros(16, np.float32)
+    p[0] = 4096.0                   # partial 0: 2^24
+    p[1:8] = 1.0                    # partials 1..7: 1 + 1
+    p[9:16] = 1.0
+    got = O.rerank_distance(p, q)
+    assert got == np.float32(2 ** 24 + 14)
+    seq = np.float32(0)
+    for x in (p - q) ** 2:
+        seq = np.float32(seq + x)
+    assert seq == np.float32(2 ** 24) and got != seq
+
+
+def test_rerank_matches_fp64_order_ties_and_padding():
+    rng = np.random.default_rng(4)
+    n, d, nq, nc = 400, 96, 30, 64
+    base = rng.integers(0, 20, size=(n, d)).astype(np.float32)  # integer data: exact distances
+    qs = rng.integers(0, 20, size=(nq, d)).astype(np.float32)
+    cand = np.stack([rng.permutation(n)[:nc] for _ in range(nq)]).astype(np.int32)
+    cand[0, 5] = cand[0, 6] = n            # padding ids sort last
+    base[cand[1, 7]] = base[cand[1, 3]]    # an exact tie: the lower id first
+    ids, dists = O.rerank(base, qs, cand, nc)
+    ex = ((base[np.minimum(cand, n - 1)].astype(np.float64) - qs[:, None, :]) ** 2).sum(-1)
+    ex[cand >= n] = np.inf
+    for i in range(nq):
+        key = sorted(zip(ex[i], cand[i]))
+        assert [c for _, c in key] == ids[i].tolist()
+        assert np.array_equal(np.array([e for e, _ in key], np.float32), dists[i])
+    assert ids[0, -1] == n and ids[0, -2] == n and np.isinf(dists[0, -1])
+    top3, _ = O.rerank(base, qs, cand, 3)
+    assert np.array_equal(top3, ids[:, :3])
+    with pytest.raises(ValueError):
+        O.rerank(base[:, :90], qs[:, :90], cand, 3)  # d % 8 != 0
