@@ -379,7 +379,13 @@ void Graph::Init_entry() {  // entry.cu:8-98
 
 void Graph::Search() {  // graph.cu:254-293
   const int qdim = algo_ == 0 ? dim_ : search_dim_;
-  Dev dq, dpq, dleaf, dids, ddist;
+  int beam = 1;
+  while (beam < n_candidates_) beam <<= 1;  // graph.cu:53
+  const int kk = reorder_ ? beam : topk_;  // REORDER: the whole beam (graph_search.cuh:1955)
+  if (reorder_)
+    check(rte_set_rerank_base(ctx_, points_.data(), np_, dim_, dim_, RTE_HOST),
+          "rte_set_rerank_base");
+  Dev dq, dpq, dleaf, dids, ddist, dcand;
   hip_ok(hipMalloc(&dq.p, (size_t)nq_ * dim_ * 4), "hipMalloc queries");
   hip_ok(hipMemcpy(dq.p, queries_.data(), (size_t)nq_ * dim_ * 4, hipMemcpyHostToDevice),
          "copy queries");
@@ -387,8 +393,9 @@ void Graph::Search() {  // graph.cu:254-293
   hip_ok(hipMalloc(&dleaf.p, (size_t)nq_ * 4), "hipMalloc leaf");
   hip_ok(hipMalloc(&dids.p, (size_t)nq_ * topk_ * 4), "hipMalloc ids");
   hip_ok(hipMalloc(&ddist.p, (size_t)nq_ * topk_ * 4), "hipMalloc dists");
+  if (reorder_) hip_ok(hipMalloc(&dcand.p, (size_t)nq_ * kk * 4), "hipMalloc candidates");
   rte_params p{};
-  p.k = topk_;
+  p.k = kk;
   p.beam = n_candidates_;
   p.width = search_width_;
   p.max_iter = max_iter_;
@@ -404,8 +411,9 @@ void Graph::Search() {  // graph.cu:254-293
   if (algo_ == 1)
     check(rte_select_entry(ctx_, sq, nq_, qdim, (int32_t*)dleaf.p, RTE_DEVICE),
           "rte_select_entry");
-  check(rte_search(ctx_, sq, nq_, qdim, (const int32_t*)dleaf.p, &p, (int32_t*)dids.p,
-                   (float*)ddist.p, nullptr, RTE_DEVICE),
+  check(rte_search(ctx_, sq, nq_, qdim, (const int32_t*)dleaf.p, &p,
+                   (int32_t*)(reorder_ ? dcand.p : dids.p), reorder_ ? nullptr : (float*)ddist.p,
+                   nullptr, RTE_DEVICE),
         "rte_search");
   check(rte_synchronize(ctx_), "sync");
   const auto t1 = std::chrono::high_resolution_clock::now();
@@ -413,6 +421,18 @@ void Graph::Search() {  // graph.cu:254-293
   std::ostringstream os;
   os << "time search: " << search_ms_ << " ms\n";  // Timing.h:41-46
   log_line(os.str());
+  if (reorder_) {  // graph.cu:326-336, on the device
+    const auto t2 = std::chrono::high_resolution_clock::now();
+    check(rte_rerank(ctx_, (const float*)dq.p, nq_, dim_, (const int32_t*)dcand.p, kk, topk_,
+                     (int32_t*)dids.p, (float*)ddist.p, RTE_DEVICE),
+          "rte_rerank");
+    check(rte_synchronize(ctx_), "sync");
+    reorder_ms_ = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t2)
+                      .count() * 1000.0;
+    std::ostringstream ro;
+    ro << "time reorder: " << reorder_ms_ << " ms\n";
+    log_line(ro.str());
+  }
   results_.resize((size_t)nq_ * topk_);
   dists_.resize((size_t)nq_ * topk_);
   hits_.resize(nq_);

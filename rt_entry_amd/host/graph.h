@@ -57,6 +57,9 @@ class Graph {
 
   // dimension of the search space (the reference's DIM); default: full d
   void set_search_dim(int dim) { search_dim_ = dim; }
+  // REORDER (graph.cu:326-336): search returns the whole beam, reranked on
+  // the original vectors to topk
+  void set_reorder(bool on) { reorder_ = on; }
   void set_out_file(const std::string& path) { out_file_ = path; }
 
   const std::vector<int32_t>& results() const { return results_; }
@@ -80,7 +83,8 @@ class Graph {
   std::vector<float> points_, queries_, pca_points_, R_, mu_, pca_queries_;
   std::vector<int32_t> gt_, graph_, hits_, results_;
   std::vector<float> dists_;
-  double recall10_ = 0.0, search_ms_ = 0.0;
+  double recall10_ = 0.0, search_ms_ = 0.0, reorder_ms_ = 0.0;
+  bool reorder_ = false;
 };
 
 }  // namespace rte_host

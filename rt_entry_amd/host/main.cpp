@@ -6,7 +6,7 @@
 //              --graph_path g.nsw --data_name cache_dir [--n_candidates 128]
 //              [--topk 0] [--max_iter 100] [--search_width 1] [--ALGO 1]
 //              [--point_ratio 0.000128] [--expand_ratio 0.2] [--dim 0]
-//              [--mode 0] [--device 0] [--ids_out ids.ivecs]
+//              [--mode 0] [--device 0] [--ids_out ids.ivecs] [--reorder 0]
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -25,7 +25,7 @@ int main(int argc, char** argv) {
       {"ALGO", "1"},             {"point_ratio", "0.000128"}, {"expand_ratio", "0.2"},
       {"max_hits", "1"},         {"n_subspaces", "1"},     {"buffer_size", "200000"},
       {"dim", "0"},              {"mode", "0"},            {"device", "0"},
-      {"ids_out", ""},           {"out_file", "out.txt"}};
+      {"ids_out", ""},           {"out_file", "out.txt"},  {"reorder", "0"}};
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     if (a.rfind("--", 0) != 0) {
@@ -52,7 +52,8 @@ int main(int argc, char** argv) {
       std::ofstream out(f["out_file"], std::ios::app);  // main.cpp:60-73
       out << "\n---------- n_candidates = " << f["n_candidates"] << "\tpoint_ratio = "
           << f["point_ratio"] << "\tmax_iter = " << f["max_iter"] << "\tgraph_path = "
-          << f["graph_path"] << "\tDIM = " << f["dim"] << "\tREORDER = 0 ----------\n\n";
+          << f["graph_path"] << "\tDIM = " << f["dim"] << "\tREORDER = " << f["reorder"]
+          << " ----------\n\n";
     }
     rte_host::Graph g(atoi(f["n_subspaces"].c_str()), atoi(f["buffer_size"].c_str()),
                       atoi(f["n_candidates"].c_str()), atoi(f["max_hits"].c_str()),
@@ -63,6 +64,7 @@ int main(int argc, char** argv) {
                       atoi(f["device"].c_str()), atoi(f["mode"].c_str()));
     g.set_out_file(f["out_file"]);
     if (atoi(f["dim"].c_str()) > 0) g.set_search_dim(atoi(f["dim"].c_str()));
+    g.set_reorder(atoi(f["reorder"].c_str()) != 0);
     g.Input();
     g.RB_Graph();
     g.Projection();

@@ -85,3 +85,29 @@ def test_driver_cold_run_builds_caches(tmp_path, int_workload):
     # a warm run reads the caches and gives the same ids
     _run(tmp_path, cache)
     assert_same(D.read_ivecs(tmp_path / "ids.ivecs"), ids, "warm run")
+
+
+def test_driver_reorder_matches_oracle(tmp_path, int_workload, ctx):
+    """--dim 64 --reorder 1 (the reference's DIM < d with REORDER): the walk
+    on the first 64 PCA columns returns the whole beam, reranked on the
+    original vectors; ids equal the oracle's search + rerank."""
+    from rt_entry_amd import data as D
+    w = int_workload
+    _write_inputs(tmp_path, w)
+    cache = tmp_path / "cache"
+    cache.mkdir()
+    D.write_fbin(cache / "O.fbin", w.R)
+    D.write_fbin(cache / "mean.fbin", w.mu[None, :])
+    D.write_fbin(cache / "pca_base.fbin", w.base)
+    D.write_aabbs(cache / "64_0.aabbs", w.boxes, w.pids)
+    _run(tmp_path, cache, ("--dim", "64", "--reorder", "1"))
+    ids = D.read_ivecs(tmp_path / "ids.ivecs")
+    ctx.set_pca(w.R, w.mu, w.R.shape[1])
+    pq = ctx.project(w.raw_queries)
+    hits = O.select_entry(w.aabbs, pq)
+    cand, _ = O.search(np.ascontiguousarray(w.base[:, :64]), w.graph, np.ascontiguousarray(pq[:, :64]),
+                       w.pids, hits, O.make_params(k=64, beam=64))
+    io, _ = O.rerank(w.raw_base, w.raw_queries, cand, 10)
+    assert_same(ids, io, "driver reorder ids")
+    text = open(tmp_path / "out.txt").read()
+    assert "REORDER = 1" in text and "time reorder:" in text
