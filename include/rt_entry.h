@@ -139,6 +139,17 @@ int rte_set_rerank_base(rte_ctx* ctx, const float* base, int64_t np, int d, int6
                         int where);
 int rte_rerank(rte_ctx* ctx, const float* Q, int64_t nq, int64_t q_stride, const int32_t* cand,
                int n_cand, int k, int32_t* ids_out, float* dists_out, int where);
+/* ---- graph construction ------------------------------------------------ */
+/* Exact k nearest neighbours of every row of X (np x d, row stride `stride`)
+ * among the rows of X, itself excluded, ordered by (squared L2 distance,
+ * id): ids_out/dists_out np x k (dists may be NULL).  FP32 MFMA distances
+ * |q|^2 + |x|^2 - 2 q.x; exact for integer-valued data with |x|^2 < 2^24.
+ * d in {32, 64, 96, 128}, k <= 64, stride % 4 == 0.  No reference
+ * counterpart: the reference loads prebuilt .nsw graphs and leaves graph
+ * building as a TODO (graph.cu:114-139). */
+int rte_knn(rte_ctx* ctx, const float* X, int64_t np, int d, int64_t stride, int k,
+            int32_t* ids_out, float* dists_out, int where);
+
 /* Batched uneven search: per-query k and beam, one persistent launch with a
  * device work queue (longest beams first), captured as a hipGraph on first
  * use and replayed while the launch shape is unchanged; no host sync.  Per

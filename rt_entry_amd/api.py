@@ -318,6 +318,23 @@ class Context:
                                         float(min_conf), C.c_void_p(out.data_ptr()), cp))
         return out
 
+    def knn(self, X, k: int, with_dists: bool = False):
+        """Exact k nearest rows of X to each row of X (itself excluded), by
+        (distance, id); X host array or device tensor."""
+        px, w, keep = _ptr(X, np.float32)
+        n, d = X.shape
+        if w == RTE_DEVICE:
+            import torch
+            ids = torch.empty((n, k), dtype=torch.int32, device=X.device)
+            dists = torch.empty((n, k), dtype=torch.float32, device=X.device) if with_dists else None
+        else:
+            ids = np.empty((n, k), np.int32)
+            dists = np.empty((n, k), np.float32) if with_dists else None
+        pi, _, _ = _ptr(ids, np.int32)
+        pd = _ptr(dists, np.float32)[0] if dists is not None else None
+        self._chk(lib().rte_knn(self._h, px, int(n), int(d), int(d), int(k), pi, pd, w))
+        return (ids, dists) if with_dists else ids
+
     def set_rerank_base(self, base):
         """Full-precision vectors for rte_rerank (np x d f32, host or device)."""
         p, w, keep = _ptr(base, np.float32)

@@ -18,6 +18,7 @@
 #include "kdtree.h"
 #include "project.h"
 #include "rerank.h"
+#include "knn.h"
 #include "search.h"
 
 namespace {
@@ -294,6 +295,44 @@ int rte_rerank(rte_ctx* c, const float* Q, int64_t nq, int64_t q_stride, const i
   if (int r = finish_out(c, ids_out, di, (size_t)nq * k, where)) return r;
   if (dists_out)
     if (int r = finish_out(c, dists_out, dd, (size_t)nq * k, where)) return r;
+  return RTE_OK;
+}
+
+int rte_knn(rte_ctx* c, const float* X, int64_t np, int d, int64_t stride, int k,
+            int32_t* ids_out, float* dists_out, int where) {
+  if (!c) return RTE_EINVAL;
+  if (!X || !ids_out || np < 2 || np >= 0x7fffffff || stride < d || stride % 4 != 0)
+    return fail(c, RTE_EINVAL, "bad knn arguments (np %lld, d %d, stride %lld)", (long long)np, d,
+                (long long)stride);
+  if (d != 32 && d != 64 && d != 96 && d != 128)
+    return fail(c, RTE_EUNSUP, "knn kernel compiled for d in {32, 64, 96, 128}, not %d", d);
+  if (k < 1 || k > rte::kKnnK || k >= np)
+    return fail(c, RTE_EINVAL, "knn needs 1 <= k <= %d and k < np", rte::kKnnK);
+  if (int r = set_device(c)) return r;
+  const float* dx;
+  if (int r = stage_in(c, c->s_in, X, (size_t)np * stride, where, &dx)) return r;
+  if (where == RTE_DEVICE && ((uintptr_t)X & 15) != 0)
+    return fail(c, RTE_EINVAL, "X not 16-byte aligned");
+  int32_t* di;
+  float* dd = nullptr;
+  if (int r = stage_out(c, c->s_out, ids_out, (size_t)np * k, where, &di)) return r;
+  if (dists_out)
+    if (int r = stage_out(c, c->s_aux2, dists_out, (size_t)np * k, where, &dd)) return r;
+  DevBuf norms;  // freed on return (hipFree waits for the device)
+  HIPCHK(c, norms.reserve((size_t)np * sizeof(float)), "hipMalloc(norms)");
+  HIPCHK(c, rte::launch_row_norms(dx, np, d, stride, norms.as<float>(), c->stream), "norms");
+  // rows in launches of 128k (short kernels)
+  const int64_t step = 1 << 17;
+  for (int64_t r0 = 0; r0 < np; r0 += step) {
+    const int64_t n = std::min<int64_t>(step, np - r0);
+    HIPCHK(c, rte::launch_knn(dx, np, d, stride, norms.as<float>(), r0, n, k, di + r0 * k,
+                              dd ? dd + r0 * k : nullptr, c->stream),
+           "knn launch");
+  }
+  if (int r = finish_out(c, ids_out, di, (size_t)np * k, where)) return r;
+  if (dists_out)
+    if (int r = finish_out(c, dists_out, dd, (size_t)np * k, where)) return r;
+  HIPCHK(c, hipStreamSynchronize(c->stream), "sync knn");  // norms is freed on return
   return RTE_OK;
 }
 
