@@ -56,11 +56,22 @@ def prepare(workload: int, scale: float, dev, n_train: int = 0, keep_raw: bool =
     pbase = G.pca_rotate_gpu(base, R, mu)
     log(f"pca {time.time() - t0:.1f}s")
     t0 = time.time()
-    knn, _ = G.knn_gpu(base, None, cfg["k_fwd"], exclude_self=True)
-    graph = G.graph_from_knn_gpu(knn, cfg["degree"], rev_rank=cfg["degree"])
+    if d in (32, 64, 96, 128) and cfg["k_fwd"] <= 64:
+        # the graph builder of this library: exact kNN on MFMA, then reverse
+        # edges and rank pruning
+        from rt_entry_amd import Context
+        with Context(0) as cx:
+            knn = cx.knn(base, cfg["k_fwd"])
+            graph = cx.graph_from_knn(knn, cfg["degree"], cfg["degree"])
+        how = "HIP kNN + graph kernels"
+    else:
+        knn, _ = G.knn_gpu(base, None, cfg["k_fwd"], exclude_self=True)
+        graph = G.graph_from_knn_gpu(knn, cfg["degree"], rev_rank=cfg["degree"])
+        how = "torch"
     del knn
     torch.cuda.synchronize()
-    log(f"knn graph {time.time() - t0:.1f}s")
+    t_graph = time.time() - t0
+    log(f"knn graph ({how}) {t_graph:.1f}s")
     t0 = time.time()
     gt, _ = G.knn_gpu(base, Q, 10, fp64=True)
     gt = gt.cpu().numpy()
@@ -74,7 +85,8 @@ def prepare(workload: int, scale: float, dev, n_train: int = 0, keep_raw: bool =
     aabbs, boxes, pids = D.build_leaves(pb_h[:, :3], cap, 0.2)
     log(f"kd-tree {boxes.shape[0]} leaves cap {cap} in {time.time() - t0:.1f}s")
     out = dict(base=pbase, base_h=pb_h, graph=graph, Q=Q, R=R, mu=mu, gt=gt, aabbs=aabbs,
-               pids=pids, n=n, d=d, cap=cap, n_leaf=boxes.shape[0], cfg=cfg)
+               pids=pids, n=n, d=d, cap=cap, n_leaf=boxes.shape[0], cfg=cfg,
+               graph_build=dict(seconds=round(t_graph, 2), how=how))
     if n_train:
         t0 = time.time()
         tr = torch.from_numpy(D.extra_queries(workload, n_train)).to(dev)
@@ -535,6 +547,7 @@ def main():
                       else "kd-tree"),
             "width": 1, "mode": "tuned" if args.mode else "ref",
             "search_dims": d_search, "rerank": bool(sd),
+            "graph_build": W["graph_build"],
             "kd_tree_entry": summary(kd),
             "kd_tree_entry_batched": summary(kd_b),
             "learned_entry_batched": summary(ln),

@@ -150,6 +150,15 @@ int rte_rerank(rte_ctx* ctx, const float* Q, int64_t nq, int64_t q_stride, const
 int rte_knn(rte_ctx* ctx, const float* X, int64_t np, int d, int64_t stride, int k,
             int32_t* ids_out, float* dists_out, int where);
 
+/* Search graph from kNN lists (np x kf, each row by (distance, id), e.g.
+ * from rte_knn): candidate edges of u are u -> knn[u][j] (rank j) and, for
+ * every v with knn[v][j] == u and j < rev_rank, u -> v (rank j); a pair keeps
+ * its smallest (rank, forward before reverse); each row keeps the `degree`
+ * candidates of smallest (rank, reverse, id), padded with np (graph_out
+ * np x degree, ready for rte_set_graph).  kf, degree <= 64. */
+int rte_graph_from_knn(rte_ctx* ctx, const int32_t* knn, int64_t np, int kf, int degree,
+                       int rev_rank, int32_t* graph_out, int where);
+
 /* Batched uneven search: per-query k and beam, one persistent launch with a
  * device work queue (longest beams first), captured as a hipGraph on first
  * use and replayed while the launch shape is unchanged; no host sync.  Per

@@ -335,6 +335,22 @@ class Context:
         self._chk(lib().rte_knn(self._h, px, int(n), int(d), int(d), int(k), pi, pd, w))
         return (ids, dists) if with_dists else ids
 
+    def graph_from_knn(self, knn, degree: int, rev_rank: int | None = None):
+        """Rank-pruned graph with reverse edges from kNN lists (host array or
+        device tensor), rows padded with np."""
+        pk, w, keep = _ptr(knn, np.int32)
+        n, kf = knn.shape
+        rev_rank = degree if rev_rank is None else rev_rank
+        if w == RTE_DEVICE:
+            import torch
+            g = torch.empty((n, degree), dtype=torch.int32, device=knn.device)
+        else:
+            g = np.empty((n, degree), np.int32)
+        pg, _, _ = _ptr(g, np.int32)
+        self._chk(lib().rte_graph_from_knn(self._h, pk, int(n), int(kf), int(degree),
+                                           int(rev_rank), pg, w))
+        return g
+
     def set_rerank_base(self, base):
         """Full-precision vectors for rte_rerank (np x d f32, host or device)."""
         p, w, keep = _ptr(base, np.float32)

@@ -336,6 +336,26 @@ int rte_knn(rte_ctx* c, const float* X, int64_t np, int d, int64_t stride, int k
   return RTE_OK;
 }
 
+int rte_graph_from_knn(rte_ctx* c, const int32_t* knn, int64_t np, int kf, int degree,
+                       int rev_rank, int32_t* graph_out, int where) {
+  if (!c) return RTE_EINVAL;
+  if (!knn || !graph_out || np < 1 || np >= 0x7fffffff || kf < 1 || kf > 64 || degree < 1 ||
+      degree > 64 || rev_rank < 0)
+    return fail(c, RTE_EINVAL, "bad graph arguments (kf %d, degree %d <= 64)", kf, degree);
+  if (int r = set_device(c)) return r;
+  const int32_t* dk;
+  int32_t* dg;
+  if (int r = stage_in(c, c->s_aux, knn, (size_t)np * kf, where, &dk)) return r;
+  if (int r = stage_out(c, c->s_out, graph_out, (size_t)np * degree, where, &dg)) return r;
+  DevBuf work;  // freed on return (hipFree waits for the device)
+  HIPCHK(c, work.reserve(rte::graph_from_knn_work_bytes(np, kf, rev_rank)), "hipMalloc(graph work)");
+  HIPCHK(c, rte::launch_graph_from_knn(dk, np, kf, degree, rev_rank, dg, work.p, c->stream),
+         "graph launch");
+  if (int r = finish_out(c, graph_out, dg, (size_t)np * degree, where)) return r;
+  HIPCHK(c, hipStreamSynchronize(c->stream), "sync graph");
+  return RTE_OK;
+}
+
 __global__ void repack_graph_kernel(const int32_t* __restrict__ src, int64_t np, int degree,
                                     int stride, int32_t* __restrict__ dst) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -192,6 +192,198 @@ __global__ __launch_bounds__(512) void knn_kernel(const float* __restrict__ X, i
   }
 }
 
+// ---- search graph from kNN lists -----------------------------------------
+// Candidate edges of u: forward u -> knn[u][j] (rank j, type 0) and reverse
+// u -> v for every v with knn[v][j] == u, j < rev_rank (rank j, type 1); a
+// pair (u, v) keeps its smallest (rank, type); row u keeps the `degree`
+// candidates of smallest (rank, type, v), padded with np.
+
+__global__ void rev_count_kernel(const int32_t* __restrict__ knn, int64_t np, int kf, int kr,
+                                 int32_t* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np * kr) return;
+  const int64_t v = i / kr;
+  const int j = (int)(i - v * kr);
+  const int32_t u = knn[v * kf + j];
+  if (u >= 0 && u < np && u != v) atomicAdd(&cnt[u], 1);
+}
+
+// exclusive scan in blocks of 1024 (block totals to `sums`), then the block
+// totals by one workgroup, then the carry-in
+__global__ __launch_bounds__(1024) void scan_blocks_kernel(const int32_t* __restrict__ in, int64_t n,
+                                                          int64_t* __restrict__ out,
+                                                          int64_t* __restrict__ sums) {
+  __shared__ int64_t sh[1024];
+  const int t = threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * 1024 + t;
+  const int64_t x = i < n ? in[i] : 0;
+  sh[t] = x;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int64_t y = t >= o ? sh[t - o] : 0;
+    __syncthreads();
+    sh[t] += y;
+    __syncthreads();
+  }
+  if (i < n) out[i] = sh[t] - x;
+  if (t == 1023) sums[blockIdx.x] = sh[t];
+}
+
+__global__ __launch_bounds__(1024) void scan_sums_kernel(int64_t* __restrict__ sums, int64_t nb,
+                                                        int64_t* __restrict__ total) {
+  __shared__ int64_t sh[1024];
+  const int t = threadIdx.x;
+  int64_t carry = 0;
+  for (int64_t b0 = 0; b0 < nb; b0 += 1024) {
+    const int64_t b = b0 + t;
+    const int64_t x = b < nb ? sums[b] : 0;
+    sh[t] = x;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int64_t y = t >= o ? sh[t - o] : 0;
+      __syncthreads();
+      sh[t] += y;
+      __syncthreads();
+    }
+    if (b < nb) sums[b] = carry + sh[t] - x;
+    const int64_t blk = sh[1023];
+    __syncthreads();
+    carry += blk;
+  }
+  if (t == 0) total[0] = carry;
+}
+
+__global__ void scan_add_kernel(int64_t* __restrict__ out, int64_t n,
+                                const int64_t* __restrict__ sums, const int64_t* __restrict__ total) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += sums[i / 1024];
+  if (i == n) out[n] = total[0];
+}
+
+__global__ void rev_fill_kernel(const int32_t* __restrict__ knn, int64_t np, int kf, int kr,
+                                const int64_t* __restrict__ off, int32_t* __restrict__ cur,
+                                int64_t* __restrict__ rev) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np * kr) return;
+  const int64_t v = i / kr;
+  const int j = (int)(i - v * kr);
+  const int32_t u = knn[v * kf + j];
+  if (u >= 0 && u < np && u != v) {
+    const int slot = atomicAdd(&cur[u], 1);
+    rev[off[u] + slot] = ((int64_t)j << 32) | (int64_t)v;  // key (rank, id)
+  }
+}
+
+// (rank, type, id) order of two candidates
+__device__ __forceinline__ bool cand_less(int r1, int t1, int v1, int r2, int t2, int v2) {
+  return r1 < r2 || (r1 == r2 && (t1 < t2 || (t1 == t2 && v1 < v2)));
+}
+
+// one wave per node u
+__global__ __launch_bounds__(256) void graph_rows_kernel(const int32_t* __restrict__ knn,
+                                                         int64_t np, int kf, int degree,
+                                                         const int64_t* __restrict__ off,
+                                                         const int64_t* __restrict__ rev,
+                                                         int32_t* __restrict__ graph) {
+  const int lane = threadIdx.x & 63;
+  const int64_t u = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (u >= np) return;  // whole waves, no workgroup barrier
+  const int32_t fv = lane < kf ? knn[u * kf + lane] : -1;
+  const bool fvalid = lane < kf && fv >= 0 && fv < np && fv != u;
+  uint64_t fdrop = 0;  // forward positions displaced by a smaller-rank reverse edge
+  constexpr int64_t kNone = 0x7fffffffffffffffLL;
+  int64_t tk = kNone;  // lane i < degree: i-th best reverse key (rank << 32 | id)
+  const int64_t r0 = off[u], r1 = off[u + 1];
+  for (int64_t c0 = r0; c0 < r1; c0 += 64) {
+    const int64_t idx = c0 + lane;
+    int64_t key = idx < r1 ? rev[idx] : kNone;
+    const int rj = (int)(key >> 32);
+    const int rv = (int)(key & 0xffffffff);
+    bool keep = key != kNone;
+    for (int p = 0; p < kf; ++p) {  // is rv also a forward neighbour, at rank p?
+      const int fp = __shfl(fv, p, 64);
+      const bool match = keep && rv == fp;
+      if (__ballot(match && p > rj)) fdrop |= 1ull << p;
+      if (match && p <= rj) keep = false;
+    }
+    uint64_t m = __ballot(keep);
+    while (m) {  // insert the survivors into the sorted top-`degree` list
+      const int src = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      const int64_t ck = __shfl(key, src, 64);
+      const bool less = lane < degree && tk < ck;
+      const int pos = __popcll(__ballot(less));
+      const int64_t prev = __shfl_up(tk, 1, 64);
+      if (pos < degree && lane >= pos && lane < degree) tk = lane == pos ? ck : prev;
+    }
+  }
+  // merge: forward survivors (rank = position, type 0) and the reverse list
+  const bool fs = fvalid && !((fdrop >> lane) & 1ull);
+  const uint64_t fmask = __ballot(fs);
+  const int nf_before = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(fmask >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)fmask, 0u));
+  const bool ts = lane < degree && tk != kNone;
+  const uint64_t tmask = __ballot(ts);
+  const int tr = (int)(tk >> 32), tv = (int)(tk & 0xffffffff);
+  // output position of each surviving element = its rank in its own list +
+  // the number of elements of the other list that precede it
+  int fpos = nf_before, tpos = lane;
+  for (int q = 0; q < 64; ++q) {
+    if ((tmask >> q) & 1ull) {  // reverse element q against this lane's forward element
+      const int qr = __shfl(tr, q, 64), qv = __shfl(tv, q, 64);
+      if (fs && cand_less(qr, 1, qv, lane, 0, fv)) ++fpos;
+    }
+    if ((fmask >> q) & 1ull) {  // forward element q against this lane's reverse element
+      const int qv = __shfl(fv, q, 64);
+      if (ts && cand_less(q, 0, qv, tr, 1, tv)) ++tpos;
+    }
+  }
+  int32_t* row = graph + u * degree;
+  const int total = __popcll(fmask) + __popcll(tmask);
+  if (lane < degree && lane >= total) row[lane] = (int32_t)np;
+  if (fs && fpos < degree) row[fpos] = fv;
+  if (ts && tpos < degree) row[tpos] = tv;
+}
+
+hipError_t launch_graph_from_knn(const int32_t* knn, int64_t np, int kf, int degree, int rev_rank,
+                                 int32_t* graph, void* work, hipStream_t st) {
+  const int kr = rev_rank < kf ? rev_rank : kf;
+  // work: cnt (np) | cur (np) | off (np + 1) | sums | total | rev
+  char* w = static_cast<char*>(work);
+  int32_t* cnt = reinterpret_cast<int32_t*>(w);
+  int32_t* cur = cnt + np;
+  int64_t* off = reinterpret_cast<int64_t*>(w + ((2 * np * 4 + 255) & ~255LL));
+  const int64_t nb = (np + 1023) / 1024;
+  int64_t* sums = off + np + 1;
+  int64_t* total = sums + nb;
+  int64_t* rev = total + 1;
+  hipError_t e = hipMemsetAsync(cnt, 0, 2 * np * sizeof(int32_t), st);
+  if (e != hipSuccess) return e;
+  const int64_t ne = np * (int64_t)kr;
+  if (kr > 0) {
+    hipLaunchKernelGGL(rev_count_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, knn,
+                       np, kf, kr, cnt);
+  }
+  hipLaunchKernelGGL(scan_blocks_kernel, dim3((unsigned)nb), dim3(1024), 0, st, cnt, np, off, sums);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, st, sums, nb, total);
+  hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)((np + 1 + 255) / 256)), dim3(256), 0, st, off,
+                     np, sums, total);
+  if (kr > 0) {
+    hipLaunchKernelGGL(rev_fill_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, knn,
+                       np, kf, kr, off, cur, rev);
+  }
+  hipLaunchKernelGGL(graph_rows_kernel, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, st, knn, np,
+                     kf, degree, off, rev, graph);
+  return hipGetLastError();
+}
+
+size_t graph_from_knn_work_bytes(int64_t np, int kf, int rev_rank) {
+  const int kr = rev_rank < kf ? rev_rank : kf;
+  const int64_t nb = (np + 1023) / 1024;
+  return (size_t)((2 * np * 4 + 255) & ~255LL) + (size_t)(np + 1 + nb + 1) * 8 +
+         (size_t)np * kr * 8;
+}
+
 hipError_t launch_row_norms(const float* X, int64_t np, int d, int64_t stride, float* norms,
                             hipStream_t st) {
   hipLaunchKernelGGL(row_norms_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, X,

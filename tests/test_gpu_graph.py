@@ -36,3 +36,32 @@ def test_knn_float_data_agrees(ctx):
     print(f"[knn] float data set agreement {same:.5f}", flush=True)
     assert same > 0.999
     assert (ids != np.arange(5000)[:, None]).all()
+
+
+@pytest.mark.parametrize("degree,rev", [(32, 32), (32, 0), (48, 48), (16, 64)])
+def test_graph_from_knn_equals_numpy(ctx, degree, rev):
+    """GPU reverse edges + dedupe + rank pruning equal data.graph_from_knn
+    row for row, on the kernel's own kNN lists and on random lists with many
+    shared neighbours (hubs)."""
+    from rt_entry_amd import data as D
+    X = _int_data(5000, 128, seed=9)
+    knn = ctx.knn(X, 64)
+    g = ctx.graph_from_knn(knn, degree, rev)
+    np.testing.assert_array_equal(g, D.graph_from_knn(knn, degree, rev_rank=rev))
+    rng = np.random.default_rng(degree + rev)
+    n = 3000
+    hub = rng.integers(0, 20, size=(n, 64))          # a few ids referenced by everyone
+    rnd = np.stack([rng.permutation(n)[:64] for _ in range(n)])
+    knn2 = np.where(rng.random((n, 64)) < 0.3, hub, rnd).astype(np.int32)
+    # distinct ids per row, self excluded (as rte_knn produces)
+    for i in range(n):
+        row, seen = [], set([i])
+        for x in list(knn2[i]) + list(rng.permutation(n)):
+            if x not in seen:
+                seen.add(x)
+                row.append(x)
+            if len(row) == 64:
+                break
+        knn2[i] = row
+    g2 = ctx.graph_from_knn(knn2, degree, rev)
+    np.testing.assert_array_equal(g2, D.graph_from_knn(knn2, degree, rev_rank=rev))
