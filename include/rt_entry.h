@@ -139,6 +139,23 @@ int rte_set_rerank_base(rte_ctx* ctx, const float* base, int64_t np, int d, int6
                         int where);
 int rte_rerank(rte_ctx* ctx, const float* Q, int64_t nq, int64_t q_stride, const int32_t* cand,
                int n_cand, int k, int32_t* ids_out, float* dists_out, int where);
+/* PCA fit: FP64 mean and covariance / (n - 1) on the GPU (two passes,
+ * fixed-order reductions), FP64 Jacobi eigensolver on the host; R_out d x d
+ * (host, column j = eigenvector of the j-th largest eigenvalue, largest-
+ * magnitude component positive), mu_out d, eig_out d (may be NULL).  X is
+ * n x d with row stride `stride` (host or device), n >= 2, d <= 4096.
+ * Deterministic.  Replaces PCA::PCA (pca.cpp:4-58). */
+int rte_pca_fit(rte_ctx* ctx, const float* X, int64_t n, int d, int64_t stride, float* R_out,
+                float* mu_out, double* eig_out, int where);
+
+/* out = (X - mu) R[:, :d_out] in FP64 (j ascending, multiply then add),
+ * stored FP32: the base rotation of pca.cpp:122-139, bit-identical to that
+ * host loop.  X n x d with row stride `stride` and out n x d_out are host or
+ * device per `where`; R (d x d_out, row-major) and mu (d) are host arrays.
+ * Synchronous. */
+int rte_pca_rotate(rte_ctx* ctx, const float* X, int64_t n, int d, int64_t stride, const float* R,
+                   const float* mu, int d_out, float* out, int where);
+
 /* ---- graph construction ------------------------------------------------ */
 /* Exact k nearest neighbours of every row of X (np x d, row stride `stride`)
  * among the rows of X, itself excluded, ordered by (squared L2 distance,

@@ -318,6 +318,41 @@ class Context:
                                         float(min_conf), C.c_void_p(out.data_ptr()), cp))
         return out
 
+    def pca_fit(self, X, d=None):
+        """(R d x d, mu d, eigenvalues d) of the first d columns of the rows of
+        X (host or device; the row stride is X's column count)."""
+        px, w, keep = _ptr(X, np.float32)
+        n, stride = X.shape
+        d = stride if d is None else d
+        R = np.empty((d, d), np.float32)
+        mu = np.empty(d, np.float32)
+        eig = np.empty(d, np.float64)
+        self._chk(lib().rte_pca_fit(self._h, px, int(n), int(d), int(stride),
+                                    R.ctypes.data_as(C.c_void_p), mu.ctypes.data_as(C.c_void_p),
+                                    eig.ctypes.data_as(C.c_void_p), w))
+        return R, mu, eig
+
+    def pca_rotate(self, X, R, mu, d_out=None):
+        """(X - mu) R[:, :d_out] in FP64, stored FP32 (torch tensor on the
+        device for a device X, numpy otherwise)."""
+        px, w, keep = _ptr(X, np.float32)
+        n, d = X.shape
+        R = np.ascontiguousarray(np.asarray(R, np.float32))
+        mu = np.ascontiguousarray(np.asarray(mu, np.float32))
+        d_out = R.shape[1] if d_out is None else int(d_out)
+        R = np.ascontiguousarray(R[:, :d_out])
+        if w == RTE_DEVICE:
+            import torch
+            out = torch.empty((n, d_out), dtype=torch.float32, device=X.device)
+            po = C.c_void_p(out.data_ptr())
+        else:
+            out = np.empty((n, d_out), np.float32)
+            po = out.ctypes.data_as(C.c_void_p)
+        self._chk(lib().rte_pca_rotate(self._h, px, int(n), int(d), int(d),
+                                       R.ctypes.data_as(C.c_void_p),
+                                       mu.ctypes.data_as(C.c_void_p), d_out, po, w))
+        return out
+
     def knn(self, X, k: int, with_dists: bool = False):
         """Exact k nearest rows of X to each row of X (itself excluded), by
         (distance, id); X host array or device tensor."""

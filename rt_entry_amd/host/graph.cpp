@@ -144,99 +144,6 @@ void write_fbin(const std::string& path, uint32_t n, uint32_t d, const float* da
   f.write((const char*)data, 4LL * n * d);
 }
 
-// cyclic Jacobi eigen-decomposition of a symmetric d x d matrix (FP64)
-static void jacobi_eigen(std::vector<double>& A, int d, std::vector<double>& V,
-                         std::vector<double>& w) {
-  V.assign((size_t)d * d, 0.0);
-  for (int i = 0; i < d; ++i) V[(size_t)i * d + i] = 1.0;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0, diag = 0.0;
-    for (int i = 0; i < d; ++i)
-      for (int j = 0; j < d; ++j)
-        (i == j ? diag : off) += A[(size_t)i * d + j] * A[(size_t)i * d + j];
-    if (off <= 1e-30 * diag || off == 0.0) break;
-    for (int p = 0; p < d - 1; ++p)
-      for (int q = p + 1; q < d; ++q) {
-        const double apq = A[(size_t)p * d + q];
-        if (fabs(apq) < 1e-300) continue;
-        const double app = A[(size_t)p * d + p], aqq = A[(size_t)q * d + q];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < d; ++k) {  // rows p, q
-          const double akp = A[(size_t)k * d + p], akq = A[(size_t)k * d + q];
-          A[(size_t)k * d + p] = c * akp - s * akq;
-          A[(size_t)k * d + q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < d; ++k) {  // columns p, q
-          const double apk = A[(size_t)p * d + k], aqk = A[(size_t)q * d + k];
-          A[(size_t)p * d + k] = c * apk - s * aqk;
-          A[(size_t)q * d + k] = s * apk + c * aqk;
-        }
-        for (int k = 0; k < d; ++k) {
-          const double vkp = V[(size_t)k * d + p], vkq = V[(size_t)k * d + q];
-          V[(size_t)k * d + p] = c * vkp - s * vkq;
-          V[(size_t)k * d + q] = s * vkp + c * vkq;
-        }
-      }
-  }
-  w.resize(d);
-  for (int i = 0; i < d; ++i) w[i] = A[(size_t)i * d + i];
-}
-
-void pca_fit(const float* X, int64_t n, int d, std::vector<float>& R, std::vector<float>& mu,
-             std::vector<double>* eigenvalues) {
-  std::vector<double> mean(d, 0.0), C((size_t)d * d, 0.0);
-#pragma omp parallel
-  {
-    std::vector<double> local(d, 0.0);
-#pragma omp for schedule(static)
-    for (int64_t i = 0; i < n; ++i)
-      for (int j = 0; j < d; ++j) local[j] += X[i * d + j];
-#pragma omp critical
-    for (int j = 0; j < d; ++j) mean[j] += local[j];
-  }
-  for (int j = 0; j < d; ++j) mean[j] /= (double)n;
-#pragma omp parallel
-  {
-    std::vector<double> local((size_t)d * d, 0.0), x(d);
-#pragma omp for schedule(static)
-    for (int64_t i = 0; i < n; ++i) {
-      for (int j = 0; j < d; ++j) x[j] = (double)X[i * d + j] - mean[j];
-      for (int a = 0; a < d; ++a)
-        for (int b = a; b < d; ++b) local[(size_t)a * d + b] += x[a] * x[b];
-    }
-#pragma omp critical
-    for (size_t k = 0; k < local.size(); ++k) C[k] += local[k];
-  }
-  for (int a = 0; a < d; ++a)
-    for (int b = a; b < d; ++b) {
-      C[(size_t)a * d + b] /= (double)std::max<int64_t>(1, n - 1);  // pca.cpp:8-9
-      C[(size_t)b * d + a] = C[(size_t)a * d + b];
-    }
-  std::vector<double> V, w;
-  jacobi_eigen(C, d, V, w);
-  std::vector<int> order(d);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
-  R.assign((size_t)d * d, 0.f);
-  for (int j = 0; j < d; ++j) {
-    const int src = order[j];
-    // deterministic sign: largest-magnitude component positive
-    int arg = 0;
-    for (int i = 1; i < d; ++i)
-      if (fabs(V[(size_t)i * d + src]) > fabs(V[(size_t)arg * d + src])) arg = i;
-    const double sg = V[(size_t)arg * d + src] < 0 ? -1.0 : 1.0;
-    for (int i = 0; i < d; ++i) R[(size_t)i * d + j] = (float)(sg * V[(size_t)i * d + src]);
-  }
-  mu.resize(d);
-  for (int j = 0; j < d; ++j) mu[j] = (float)mean[j];
-  if (eigenvalues) {
-    eigenvalues->resize(d);
-    for (int j = 0; j < d; ++j) (*eigenvalues)[j] = w[order[j]];
-  }
-}
-
 Graph::Graph(int /*n_subspaces*/, int /*buffer_size*/, int n_candidates, int /*max_hits*/,
              double expand_ratio, double point_ratio, std::string data_name,
              std::string& data_path, std::string& query_path, std::string& gt_path,
@@ -309,7 +216,11 @@ void Graph::Projection() {  // graph.cu:141-252, pca.cpp
     read_data(mean, n1, d1, mu_);
     if (n1 != 1 || d1 != dim_) throw Error("mean shape mismatch in " + mean);
   } else {
-    pca_fit(points_.data(), np_, dim_, R_, mu_);
+    R_.resize((size_t)dim_ * dim_);
+    mu_.resize(dim_);
+    check(rte_pca_fit(ctx_, points_.data(), np_, dim_, dim_, R_.data(), mu_.data(), nullptr,
+                      RTE_HOST),
+          "rte_pca_fit");
     mkdir(data_name_.c_str(), 0755);
     write_fbin(rot, dim_, dim_, R_.data());
     write_fbin(mean, 1, dim_, mu_.data());
@@ -320,14 +231,9 @@ void Graph::Projection() {  // graph.cu:141-252, pca.cpp
     if (n1 != np_ || d1 != dim_) throw Error("pca_base shape mismatch in " + pb);
   } else {
     pca_points_.resize((size_t)np_ * dim_);
-#pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < np_; ++i)
-      for (int c = 0; c < dim_; ++c) {
-        double s = 0.0;
-        for (int j = 0; j < dim_; ++j)
-          s += ((double)points_[i * dim_ + j] - mu_[j]) * (double)R_[(size_t)j * dim_ + c];
-        pca_points_[i * dim_ + c] = (float)s;
-      }
+    check(rte_pca_rotate(ctx_, points_.data(), np_, dim_, dim_, R_.data(), mu_.data(), dim_,
+                         pca_points_.data(), RTE_HOST),
+          "rte_pca_rotate");
     write_fbin(pb, np_, dim_, pca_points_.data());
   }
   // search space: the first search_dim_ PCA columns, row stride dim_

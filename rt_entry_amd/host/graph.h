@@ -32,11 +32,6 @@ void read_ivecs(const std::string& path, int& n, int& d, std::vector<int32_t>& o
 void read_graph(const std::string& path, int n, int& degree, std::vector<int32_t>& out);
 void write_fbin(const std::string& path, uint32_t n, uint32_t d, const float* data);
 
-// pca.cpp:4-58: FP64 mean and covariance, eigenvectors by descending
-// eigenvalue (cyclic Jacobi); R is d x d row-major, column j = eigenvector j
-void pca_fit(const float* X, int64_t n, int d, std::vector<float>& R, std::vector<float>& mu,
-             std::vector<double>* eigenvalues = nullptr);
-
 class Graph {
  public:
   Graph(int n_subspaces, int buffer_size, int n_candidates, int max_hits, double expand_ratio,

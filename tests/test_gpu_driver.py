@@ -64,7 +64,7 @@ def test_driver_matches_oracle_on_cached_artefacts(tmp_path, int_workload, ctx):
     assert abs(rec - O.recall(ids, w.gt)) < 1e-6
 
 
-def test_driver_cold_run_builds_caches(tmp_path, int_workload):
+def test_driver_cold_run_builds_caches(tmp_path, int_workload, ctx):
     from rt_entry_amd import data as D
     w = int_workload
     _write_inputs(tmp_path, w)
@@ -74,8 +74,13 @@ def test_driver_cold_run_builds_caches(tmp_path, int_workload):
     assert np.allclose(R.T @ R, np.eye(R.shape[0]), atol=1e-4)
     # same principal subspace as the FP64 numpy fit (eigenvectors up to sign)
     assert np.allclose(np.abs(R[:, :3].T @ w.R[:, :3].astype(np.float64)).diagonal(), 1, atol=1e-3)
+    # the cold run fits and rotates with the library's GPU PCA
+    R_lib, mu_lib, _ = ctx.pca_fit(w.raw_base)
+    assert_same(D.read_fbin(cache / "O.fbin"), R_lib, "cached rotation")
+    assert_same(D.read_fbin(cache / "mean.fbin")[0], mu_lib, "cached mean")
     pb = D.read_fbin(cache / "pca_base.fbin")
     assert pb.shape == w.base.shape
+    assert_same(pb, ctx.pca_rotate(w.raw_base, R_lib, mu_lib), "cached pca base")
     boxes, pids = D.read_aabbs(cache / "64_0.aabbs")
     exp_boxes, belong = O.kdtree_build(pb[:, :3], 64)
     assert_same(boxes, exp_boxes, "cached boxes")

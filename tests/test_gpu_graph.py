@@ -65,3 +65,66 @@ def test_graph_from_knn_equals_numpy(ctx, degree, rev):
         knn2[i] = row
     g2 = ctx.graph_from_knn(knn2, degree, rev)
     np.testing.assert_array_equal(g2, D.graph_from_knn(knn2, degree, rev_rank=rev))
+
+
+def _aniso(n, d, seed):
+    """Gaussian rows with distinct variances along a random orthogonal basis
+    (well separated eigenvalues, so the eigenvectors are unique up to sign)."""
+    rng = np.random.default_rng(seed)
+    Qb, _ = np.linalg.qr(rng.normal(size=(d, d)))
+    scales = np.linspace(1.0, 12.0, d)[::-1]
+    return (rng.normal(size=(n, d)) * scales @ Qb.T + rng.uniform(-5, 5, size=d)).astype(
+        np.float32)
+
+
+@pytest.mark.parametrize("n,d", [(20000, 128), (7001, 96), (5000, 48), (3000, 13), (6000, 200)])
+def test_pca_fit_equals_numpy(ctx, n, d):
+    """rte_pca_fit (GPU FP64 covariance + host Jacobi) against data.pca_fit
+    (numpy FP64 covariance + LAPACK eigh): same mean, eigenvalues, ordering
+    and sign rule."""
+    from rt_entry_amd import data as D
+    X = _aniso(n, d, seed=d)
+    print(f"[pca] n={n} d={d}", flush=True)
+    R, mu, eig = ctx.pca_fit(X)
+    R0, mu0, eig0 = D.pca_fit(X)
+    np.testing.assert_allclose(mu, mu0, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(eig, eig0, rtol=1e-9)
+    np.testing.assert_allclose(R, R0, atol=2e-5)
+    # orthonormal columns
+    np.testing.assert_allclose(R.astype(np.float64).T @ R, np.eye(d), atol=1e-5)
+
+
+def test_pca_fit_strided_device_input(ctx):
+    import torch
+    from rt_entry_amd import data as D
+    X = _aniso(9000, 64, seed=3)
+    wide = np.concatenate([X, np.full((9000, 16), 1e6, np.float32)], axis=1)
+    R, mu, eig = ctx.pca_fit(torch.from_numpy(wide).cuda(), d=64)
+    R0, mu0, eig0 = D.pca_fit(X)
+    np.testing.assert_allclose(mu, mu0, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(eig, eig0, rtol=1e-9)
+    np.testing.assert_allclose(R, R0, atol=2e-5)
+
+
+def _rotate_ordered(X, R, mu):
+    """(X - mu) R in FP64 with the reduction over j ascending, multiply then
+    add -- the operation order of the reference's host loop."""
+    X64 = X.astype(np.float64)
+    R64 = R.astype(np.float64)
+    s = np.zeros((X.shape[0], R.shape[1]))
+    for j in range(X.shape[1]):
+        s = s + (X64[:, j] - np.float64(mu[j]))[:, None] * R64[j][None, :]
+    return s.astype(np.float32)
+
+
+@pytest.mark.parametrize("n,d,d_out,device", [(5000, 128, 128, False), (5003, 128, 64, True),
+                                              (777, 200, 200, True), (300, 13, 5, False)])
+def test_pca_rotate_bit_exact(ctx, n, d, d_out, device):
+    import torch
+    X = _aniso(n, d, seed=n + d)
+    R, mu, _ = ctx.pca_fit(X)
+    print(f"[pca_rotate] n={n} d={d} d_out={d_out} device={device}", flush=True)
+    out = ctx.pca_rotate(torch.from_numpy(X).cuda() if device else X, R, mu, d_out)
+    if device:
+        out = out.cpu().numpy()
+    np.testing.assert_array_equal(out, _rotate_ordered(X, R[:, :d_out], mu))
