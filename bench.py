@@ -52,14 +52,16 @@ def prepare(workload: int, scale: float, dev, n_train: int = 0, keep_raw: bool =
     base = torch.from_numpy(base_h).to(dev)
     Q = torch.from_numpy(q_h).to(dev)
     t0 = time.time()
-    R, mu = G.pca_fit_gpu(base)
-    pbase = G.pca_rotate_gpu(base, R, mu)
-    log(f"pca {time.time() - t0:.1f}s")
+    from rt_entry_amd import Context
+    with Context(0) as cx:  # FP64 covariance kernels + host Jacobi, FP64 rotation
+        R, mu, _ = cx.pca_fit(base)
+        pbase = cx.pca_rotate(base, R, mu)
+    t_pca = time.time() - t0
+    log(f"pca {t_pca:.2f}s")
     t0 = time.time()
     if d in (32, 64, 96, 128) and cfg["k_fwd"] <= 64:
         # the graph builder of this library: exact kNN on MFMA, then reverse
         # edges and rank pruning
-        from rt_entry_amd import Context
         with Context(0) as cx:
             knn = cx.knn(base, cfg["k_fwd"])
             graph = cx.graph_from_knn(knn, cfg["degree"], cfg["degree"])
@@ -81,12 +83,18 @@ def prepare(workload: int, scale: float, dev, n_train: int = 0, keep_raw: bool =
     # entry.cu:22: aabb_size = np * point_ratio (float32); scaled runs keep
     # the leaf count
     cap = leaf_capacity(n, cfg["point_ratio"] / scale)
+    with Context(0) as cx:  # GPU kd-tree over the first three PCA columns
+        boxes, pids = cx.build_kdtree(pbase, cap)
+    boxes, pids = boxes.cpu().numpy(), pids.cpu().numpy()
+    from rt_entry_amd import expand_aabbs
+    aabbs = expand_aabbs(boxes, 0.2)  # aabb.cu:27-38, RT_Entry::BlockUp's ratio
+    t_kd = time.time() - t0
     pb_h = pbase.cpu().numpy()
-    aabbs, boxes, pids = D.build_leaves(pb_h[:, :3], cap, 0.2)
-    log(f"kd-tree {boxes.shape[0]} leaves cap {cap} in {time.time() - t0:.1f}s")
+    log(f"kd-tree {boxes.shape[0]} leaves cap {cap} in {t_kd:.3f}s")
     out = dict(base=pbase, base_h=pb_h, graph=graph, Q=Q, R=R, mu=mu, gt=gt, aabbs=aabbs,
                pids=pids, n=n, d=d, cap=cap, n_leaf=boxes.shape[0], cfg=cfg,
-               graph_build=dict(seconds=round(t_graph, 2), how=how))
+               graph_build=dict(seconds=round(t_graph, 2), how=how),
+               prep_seconds=dict(pca_fit_and_rotate=round(t_pca, 3), kdtree=round(t_kd, 4)))
     if n_train:
         t0 = time.time()
         tr = torch.from_numpy(D.extra_queries(workload, n_train)).to(dev)
@@ -548,6 +556,7 @@ def main():
             "width": 1, "mode": "tuned" if args.mode else "ref",
             "search_dims": d_search, "rerank": bool(sd),
             "graph_build": W["graph_build"],
+            "prep_seconds": W["prep_seconds"],
             "kd_tree_entry": summary(kd),
             "kd_tree_entry_batched": summary(kd_b),
             "learned_entry_batched": summary(ln),

@@ -108,6 +108,14 @@ int rte_set_leaves(rte_ctx* ctx, const float* aabbs, const int32_t* pids, int n_
  * RT_Entry::BlockUp (entry.cu:8-77). */
 int rte_build_kdtree(const float* pts3, int64_t np, int max_node, int max_leaves,
                      float* aabbs_out, int32_t* pids_out, int* n_leaf_out);
+/* The same kd-tree built on the GPU: identical leaf sets, boxes (as values),
+ * DFS leaf order, pid table and error codes as rte_build_kdtree.  pts holds
+ * np rows of `stride` floats whose first three are the coordinates (e.g. the
+ * PCA base itself); pts, aabbs_out and pids_out are host or device per
+ * `where`.  max_node <= 2048.  Synchronous. */
+int rte_build_kdtree_gpu(rte_ctx* ctx, const float* pts, int64_t np, int64_t stride, int max_node,
+                         int max_leaves, float* aabbs_out, int32_t* pids_out, int* n_leaf_out,
+                         int where);
 /* expandAabb (aabb.cu:27-38), host memory. */
 void rte_expand_aabbs(float* aabbs, int n_leaf, float ratio);
 /* hits[q] = smallest leaf whose box holds (PQ[q*stride+0..2]), 0 if none.

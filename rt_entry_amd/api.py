@@ -353,6 +353,29 @@ class Context:
                                        mu.ctypes.data_as(C.c_void_p), d_out, po, w))
         return out
 
+    def build_kdtree(self, X, max_node: int, max_leaves: int | None = None):
+        """GPU kd-tree over the first three columns of X (host or device):
+        (unexpanded boxes [n_leaf,6], pids [n_leaf,max_node]) as numpy
+        arrays for a host X, device tensors for a device X."""
+        px, w, keep = _ptr(X, np.float32)
+        n, stride = X.shape
+        if max_leaves is None:
+            max_leaves = max(16, 4 * (n // max(1, max_node)) + 16)
+        nl = C.c_int()
+        if w == RTE_DEVICE:
+            import torch
+            aabbs = torch.empty((max_leaves, 6), dtype=torch.float32, device=X.device)
+            pids = torch.empty((max_leaves, max_node), dtype=torch.int32, device=X.device)
+            pa, pp = C.c_void_p(aabbs.data_ptr()), C.c_void_p(pids.data_ptr())
+        else:
+            aabbs = np.zeros((max_leaves, 6), np.float32)
+            pids = np.zeros((max_leaves, max_node), np.int32)
+            pa, pp = aabbs.ctypes.data_as(C.c_void_p), pids.ctypes.data_as(C.c_void_p)
+        self._chk(lib().rte_build_kdtree_gpu(self._h, px, int(n), int(stride), int(max_node),
+                                             int(max_leaves), pa, pp, C.byref(nl), w))
+        return aabbs[: nl.value].clone() if w == RTE_DEVICE else aabbs[: nl.value].copy(), \
+            pids[: nl.value].clone() if w == RTE_DEVICE else pids[: nl.value].copy()
+
     def knn(self, X, k: int, with_dists: bool = False):
         """Exact k nearest rows of X to each row of X (itself excluded), by
         (distance, id); X host array or device tensor."""

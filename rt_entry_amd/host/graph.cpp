@@ -261,14 +261,21 @@ void Graph::Init_entry() {  // entry.cu:8-98
     f.read((char*)pids.data(), 4LL * n_leaf * cap);
     if (!f) throw Error("short read in " + cache);
   } else {
-    std::vector<float> pts3((size_t)np_ * 3);
-    for (int64_t i = 0; i < np_; ++i)
-      for (int a = 0; a < 3; ++a) pts3[i * 3 + a] = pca_points_[i * dim_ + a];
     const int max_leaves = 4 * (np_ / cap) + 16;
     boxes.resize((size_t)max_leaves * 6);
     pids.resize((size_t)max_leaves * cap);
-    check(rte_build_kdtree(pts3.data(), np_, cap, max_leaves, boxes.data(), pids.data(), &n_leaf),
-          "rte_build_kdtree");
+    if (cap <= 2048) {  // GPU build over the first three PCA columns
+      check(rte_build_kdtree_gpu(ctx_, pca_points_.data(), np_, dim_, cap, max_leaves,
+                                 boxes.data(), pids.data(), &n_leaf, RTE_HOST),
+            "rte_build_kdtree_gpu");
+    } else {
+      std::vector<float> pts3((size_t)np_ * 3);
+      for (int64_t i = 0; i < np_; ++i)
+        for (int a = 0; a < 3; ++a) pts3[i * 3 + a] = pca_points_[i * dim_ + a];
+      check(rte_build_kdtree(pts3.data(), np_, cap, max_leaves, boxes.data(), pids.data(),
+                             &n_leaf),
+            "rte_build_kdtree");
+    }
     boxes.resize((size_t)n_leaf * 6);
     pids.resize((size_t)n_leaf * cap);
     mkdir(data_name_.c_str(), 0755);
