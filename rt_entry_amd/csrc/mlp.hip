@@ -623,6 +623,7 @@ struct Mlp {
   DevBuf P2, parts;        // inference: H2 (bf16) and softmax partials
   bool wt_dirty = true;
   int pcap = 0;
+  int head_slots = 0;      // head workgroups resident at once on the device
   int cap = 0;
   int step = 0;
 };
@@ -1005,15 +1006,12 @@ int predict_impl(rte_ctx* c, const float* X, int64_t ldx, int64_t n, const int32
          "padded bias");
     m->wt_dirty = false;
   }
-  const int cg = 1024;  // output columns per head workgroup
-  const int G = (d3 + cg - 1) / cg;
   const int chunk = 65536;
   const int b0 = (int)std::min<int64_t>(n, chunk);
   if (b0 > m->pcap) {
     MCHK(c, m->P2.reserve((size_t)b0 * d2 * 2), "hidden activations");
     m->pcap = b0;
   }
-  MCHK(c, m->parts.reserve((size_t)b0 * G * sizeof(rte::SoftPart)), "softmax partials");
   const int cbw = std::max((d1 + 63) / 64, (d2 + 63) / 64);
   const int cb = cbw <= 1 ? 1 : cbw <= 2 ? 2 : cbw <= 4 ? 4 : 8;
   const int ks = d2 / 32;
@@ -1032,6 +1030,24 @@ int predict_impl(rte_ctx* c, const float* X, int64_t ldx, int64_t n, const int32
        "head kernel LDS size");
   MCHK(c, hipFuncSetAttribute(hk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_h),
        "hidden kernel LDS size");
+  if (m->head_slots == 0) {
+    int cus = 0, per_cu = 0;
+    MCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device),
+         "CU count");
+    MCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ok, 256, lds_o),
+         "head kernel occupancy");
+    m->head_slots = std::max(1, cus) * std::max(1, per_cu);
+  }
+  // Output columns per head workgroup: split the columns into as many
+  // groups as keep every workgroup of a chunk resident at once (a second
+  // round of workgroups on some CUs would nearly double the kernel's time),
+  // each group a whole number of 64-column tiles.
+  const int ntile = (d3 + 63) / 64;
+  const int64_t rblk = (b0 + 127) / 128;
+  const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(ntile, m->head_slots / rblk));
+  const int cg = (ntile + gw - 1) / gw * 64;
+  const int G = (d3 + cg - 1) / cg;
+  MCHK(c, m->parts.reserve((size_t)b0 * G * sizeof(rte::SoftPart)), "softmax partials");
   if (sel_pq) {  // kd-tree leaves on the side stream, overlapping the network
     if (!c->side) {
       MCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking), "side stream");
