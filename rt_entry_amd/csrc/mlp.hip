@@ -151,9 +151,11 @@ __device__ __forceinline__ uint32_t pack2(float lo, float hi) {
 // The A operands come from LDS; the W^T fragments of four 32-wide k steps
 // are loaded from global memory before their MFMAs, so one memory latency
 // covers four steps.  CB = 16-column blocks per wave (columns >= d1 / d2 are
-// computed from zero weights and not stored).  Input rows are ldx floats
-// apart.  Requires d0 % 32 == 0, ldx % 4 == 0 and d1 % 32 == 0; dynamic LDS
-// 32 * ((d0 + 8) + (d1 + 8)) * 2 bytes.
+// computed but never stored).  Input rows are ldx floats
+// apart.  Requires d0 % 32 == 0, ldx % 4 == 0, d1 % 32 == 0 and d2 % 32 == 0;
+// dynamic LDS 32 * max((d0 + 8) + (d1 + 8), d2 + 8) * 2 bytes.  The H2 tile
+// leaves through LDS as whole rows, so every global store is 16 bytes wide
+// and no store waits on another.
 template <int CB>
 struct WFrag {
   uint4 b[4][CB];  // W^T fragments of four 32-wide k steps
@@ -163,14 +165,18 @@ struct WFrag {
 template <int CB>
 __device__ __forceinline__ void load_wfrag(WFrag<CB>& f, const uint16_t* __restrict__ WT, int N,
                                            int K, int k0, int wave, int r, int g) {
+  // every load is issued, clamped to a valid address, so the group is one
+  // run of loads with no branches between them: fragments of k steps >= K
+  // are never multiplied (hidden_layer skips those steps) and columns >= N
+  // give outputs that are never stored
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int kk = k0 + 32 * u + 8 * g;
 #pragma unroll
     for (int c = 0; c < CB; ++c) {
       const int col = 16 * (wave + 4 * c) + r;
-      f.b[u][c] = make_uint4(0, 0, 0, 0);
-      if (col < N && kk < K) f.b[u][c] = *reinterpret_cast<const uint4*>(WT + (int64_t)col * K + kk);
+      f.b[u][c] = *reinterpret_cast<const uint4*>(WT + (int64_t)min(col, N - 1) * K +
+                                                  min(kk, K - 8));
     }
   }
 }
@@ -215,7 +221,7 @@ __global__ __launch_bounds__(256) void mlp_hidden_kernel(
     const float* __restrict__ b1, int d1, const uint16_t* __restrict__ W2T,
     const float* __restrict__ b2, int d2, uint16_t* __restrict__ H2) {
   extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
-  const int ld0 = d0 + 8, ld1 = d1 + 8;
+  const int ld0 = d0 + 8, ld1 = d1 + 8, ld2 = d2 + 8;
   uint16_t* Xs = lds;
   uint16_t* H1s = lds + 32 * ld0;
   const int t = threadIdx.x;
@@ -224,6 +230,15 @@ __global__ __launch_bounds__(256) void mlp_hidden_kernel(
   const int64_t m0 = (int64_t)blockIdx.x * 32;
   WFrag<CB> w;  // first W1 group, in flight while the input tile is built
   load_wfrag<CB>(w, W1T, d1, d0, 0, wave, r, g);
+  // biases of this lane's columns, loaded once with the first group (columns
+  // past the layer width read a valid entry and are never stored)
+  float bias1[CB], bias2[CB];
+#pragma unroll
+  for (int c = 0; c < CB; ++c) {
+    const int col = 16 * (wave + 4 * c) + r;
+    bias1[c] = b1[min(col, d1 - 1)];
+    bias2[c] = b2[min(col, d2 - 1)];
+  }
   {  // normalised bf16 input tile: 8 threads per row, float4 each
     const int row = t >> 3;
     const bool in = m0 + row < n;
@@ -248,27 +263,35 @@ __global__ __launch_bounds__(256) void mlp_hidden_kernel(
   for (int c = 0; c < CB; ++c) {
     const int col = 16 * (wave + 4 * c) + r;
     if (col >= d1) continue;
-    const float bc = b1[col];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        H1s[(16 * rb + 4 * g + v) * ld1 + col] = f2bf(fmaxf(acc[rb][c][v] + bc, 0.f));
+        H1s[(16 * rb + 4 * g + v) * ld1 + col] = f2bf(fmaxf(acc[rb][c][v] + bias1[c], 0.f));
   }
   __syncthreads();
   hidden_layer<CB>(H1s, ld1, d1, W2T, d2, wave, r, g, w, acc);
+  // H2 tile through LDS (the input and H1 tiles are dead once every wave is
+  // past layer 2), then whole rows to global memory in 16-byte chunks
+  __syncthreads();
+  uint16_t* Hs = lds;
 #pragma unroll
   for (int c = 0; c < CB; ++c) {
     const int col = 16 * (wave + 4 * c) + r;
     if (col >= d2) continue;
-    const float bc = b2[col];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int64_t row = m0 + 16 * rb + 4 * g + v;
-        if (row < n) H2[row * d2 + col] = f2bf(fmaxf(acc[rb][c][v] + bc, 0.f));
-      }
+      for (int v = 0; v < 4; ++v)
+        Hs[(16 * rb + 4 * g + v) * ld2 + col] = f2bf(fmaxf(acc[rb][c][v] + bias2[c], 0.f));
+  }
+  __syncthreads();
+  const int cpr = d2 >> 3;  // 16-byte chunks per row
+  for (int e = t; e < 32 * cpr; e += 256) {
+    const int row = e / cpr, ch = e - row * cpr;
+    if (m0 + row < n)
+      *reinterpret_cast<uint4*>(H2 + (m0 + row) * d2 + 8 * ch) =
+          *reinterpret_cast<const uint4*>(&Hs[row * ld2 + 8 * ch]);
   }
 }
 
@@ -372,6 +395,11 @@ __global__ __launch_bounds__(256, 2) void mlp_head_kernel(const uint16_t* __rest
       ix[rb][v] = 0x7fffffff;
     }
   const int ntile = (c1 - c0 + 63) >> 6;
+  // bias of the lane's four columns of the current and of the next tile
+  // (padded to whole tiles with -inf)
+  float bcur[4], bnext[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) bcur[j] = bias[c0 + 16 * j + r];
 #pragma unroll
   for (int i = 0; i < KS; ++i)
     pre[i] = *reinterpret_cast<const uint4*>(wt_t + (int64_t)(c0 + COLS_PER_STEP * i) * K);
@@ -388,13 +416,16 @@ __global__ __launch_bounds__(256, 2) void mlp_head_kernel(const uint16_t* __rest
 #pragma unroll
     for (int i = 0; i < KS; ++i)
       pre[i] = *reinterpret_cast<const uint4*>(wt_t + (int64_t)(nn + COLS_PER_STEP * i) * K);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bnext[j] = bias[nn + 16 * j + r];
+    // (the compiler issues these loads during the softmax below; nothing
+    // waits on them before the LDS store at the end of the tile, and the
+    // accumulators start from the bias fetched one tile earlier)
     floatx4 acc[2][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const float bj = bias[n0 + 16 * j + r];  // padded to whole tiles with -inf
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) acc[rb][j] = floatx4{bj, bj, bj, bj};
-    }
+      for (int rb = 0; rb < 2; ++rb) acc[rb][j] = floatx4{bcur[j], bcur[j], bcur[j], bcur[j]};
     {  // B fragments of k step s + 1 are read from LDS while step s multiplies
       const uint16_t* bb = &Bs[(buf * 64 + r) * LDB + 8 * g];
       uint4 bc[4], bn[4];
@@ -426,6 +457,8 @@ __global__ __launch_bounds__(256, 2) void mlp_head_kernel(const uint16_t* __rest
 #pragma unroll
     for (int i = 0; i < KS; ++i)
       *reinterpret_cast<uint4*>(bs_t + ((buf ^ 1) * 64 + COLS_PER_STEP * i) * LDB) = pre[i];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bcur[j] = bnext[j];
     __syncthreads();
   }
   // merge the 16 lanes of each row (lanes differing in bits 0..3)
@@ -1015,7 +1048,7 @@ int predict_impl(rte_ctx* c, const float* X, int64_t ldx, int64_t n, const int32
   const int cbw = std::max((d1 + 63) / 64, (d2 + 63) / 64);
   const int cb = cbw <= 1 ? 1 : cbw <= 2 ? 2 : cbw <= 4 ? 4 : 8;
   const int ks = d2 / 32;
-  const size_t lds_h = (size_t)32 * ((d0 + 8) + (d1 + 8)) * 2;
+  const size_t lds_h = (size_t)32 * std::max((d0 + 8) + (d1 + 8), d2 + 8) * 2;
   const size_t lds_o = (size_t)2 * 64 * (d2 + 8) * 2;
   const void* hk = cb == 1   ? (const void*)rte::mlp_hidden_kernel<1>
                    : cb == 2 ? (const void*)rte::mlp_hidden_kernel<2>
