@@ -173,7 +173,8 @@ def scale_output(ctx: Context, factor: float):
     ctx.mlp_set(W, b)
 
 
-def fit_temperature(ctx: Context, validate, factors=(1.0, 1.5, 2.0, 3.0, 4.0, 6.0, 8.0)):
+def fit_temperature(ctx: Context, validate,
+                    factors=(1.0, 1.5, 2.0, 3.0, 4.0, 6.0, 8.0, 12.0, 16.0)):
     """Choose the output scale (inverse softmax temperature) that maximises
     `validate()` -- end-to-end recall on held-out queries with the fallback
     rule applied -- and leave the network scaled by it.  Returns (factor,
