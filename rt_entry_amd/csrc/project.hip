@@ -12,7 +12,8 @@
 // chunk of R's 64 columns is staged into LDS transposed ([col][k], one
 // float4 per operand read); its loads and each lane's Q fragments of the
 // chunk (one float4 per 16 k) are all issued before any is used, so a chunk
-// costs one memory round trip.
+// costs one memory round trip.  The output tile leaves through LDS as
+// 16-byte row segments.
 // Inside a 16-wide k step the MFMA's k index g maps to k0 + 4g + c for
 // component c of the lane's float4; that only permutes the k sum.
 #include <hip/hip_runtime.h>
@@ -27,6 +28,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr int kProjChunk = 128;
 constexpr int kProjLds = kProjChunk + 4;  // row pitch of the transposed chunk (floats)
 
+template <bool VEC>  // VEC: d % 4 == 0, query rows are read as float4
 __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ Q, int64_t nq,
                                                       int d, const float* __restrict__ R,
                                                       int d_out, const float* __restrict__ bias,
@@ -41,37 +43,55 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
   const int g = lane >> 4;    // k group
   const int64_t arow = row0 + r16;
   const bool arow_ok = arow < nq;
-  const bool vec_ok = (d & 3) == 0;
+  const float* qrow = Q + (arow_ok ? arow : nq - 1) * d;  // clamped: always a valid row
   floatx4 acc[4];
+  float bc[4];  // bias of the lane's four columns, loaded once up front
 #pragma unroll
-  for (int u = 0; u < 4; ++u) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int u = 0; u < 4; ++u) {
+    acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+    bc[u] = bias[min(col0 + 16 * u + r16, d_out - 1)];
+  }
   for (int kc = 0; kc < d; kc += kProjChunk) {
     // R[kc .. kc+128)[col0 .. col0+64) -> Rt[col][k]: thread t handles column
-    // t % 64 and k = kc + t / 64 + 4 j; all R and Q loads of the chunk are
-    // issued before the first is used
+    // t % 64 and k = kc + t / 64 + 4 j.  Every load of the chunk (R and this
+    // lane's Q fragments) is issued at a clamped, valid address and masked
+    // afterwards, with no branches, so all of them are in flight together and
+    // are consumed before the barrier: a chunk costs one memory round trip.
     const int rc = t & 63, k_first = t >> 6;
     const int col = col0 + rc;
+    const int colc = min(col, d_out - 1);
     float rv[kProjChunk / 4];
 #pragma unroll
     for (int j = 0; j < kProjChunk / 4; ++j) {
       const int k = kc + k_first + 4 * j;
-      rv[j] = (k < d && col < d_out) ? R[(int64_t)k * d_out + col] : 0.f;
+      rv[j] = R[(int64_t)min(k, d - 1) * d_out + colc];
     }
     float4 qa[kProjChunk / 16];
 #pragma unroll
     for (int s = 0; s < kProjChunk / 16; ++s) {
       const int ka = kc + 16 * s + 4 * g;
-      if (arow_ok && vec_ok && ka + 3 < d) {
-        qa[s] = *reinterpret_cast<const float4*>(Q + arow * d + ka);
+      if (VEC) {
+        qa[s] = *reinterpret_cast<const float4*>(qrow + min(ka, d - 4));
       } else {
-        qa[s].x = (arow_ok && ka + 0 < d) ? Q[arow * d + ka + 0] : 0.f;
-        qa[s].y = (arow_ok && ka + 1 < d) ? Q[arow * d + ka + 1] : 0.f;
-        qa[s].z = (arow_ok && ka + 2 < d) ? Q[arow * d + ka + 2] : 0.f;
-        qa[s].w = (arow_ok && ka + 3 < d) ? Q[arow * d + ka + 3] : 0.f;
+        qa[s].x = qrow[min(ka + 0, d - 1)];
+        qa[s].y = qrow[min(ka + 1, d - 1)];
+        qa[s].z = qrow[min(ka + 2, d - 1)];
+        qa[s].w = qrow[min(ka + 3, d - 1)];
       }
     }
 #pragma unroll
-    for (int j = 0; j < kProjChunk / 4; ++j) Rt[rc * kProjLds + k_first + 4 * j] = rv[j];
+    for (int j = 0; j < kProjChunk / 4; ++j) {
+      const int k = kc + k_first + 4 * j;
+      Rt[rc * kProjLds + k_first + 4 * j] = (k < d && col < d_out) ? rv[j] : 0.f;
+    }
+#pragma unroll
+    for (int s = 0; s < kProjChunk / 16; ++s) {
+      const int ka = kc + 16 * s + 4 * g;
+      qa[s].x = (arow_ok && ka + 0 < d) ? qa[s].x : 0.f;
+      qa[s].y = (arow_ok && ka + 1 < d) ? qa[s].y : 0.f;
+      qa[s].z = (arow_ok && ka + 2 < d) ? qa[s].z : 0.f;
+      qa[s].w = (arow_ok && ka + 3 < d) ? qa[s].w : 0.f;
+    }
     __syncthreads();
     const int steps = min(kProjChunk, d - kc + 15) / 16;
 #pragma unroll
@@ -91,16 +111,31 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
     }
     __syncthreads();
   }
-  // D layout: lane holds D[4*g + i][r16]
+  // D layout: lane holds D[4*g + i][r16].  The 64 x 64 tile goes through LDS
+  // (the R chunk buffer is free after the last barrier) and leaves as whole
+  // row segments of 16-byte stores: single-float stores, each under its own
+  // bounds branch, were each made to wait for the previous store.
+  float* T = Rt;  // [64][kProjLds]
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int col = col0 + 16 * u + r16;
-    if (col >= d_out) continue;
-    const float bc = bias[col];
+  for (int u = 0; u < 4; ++u)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int64_t row = row0 + 4 * g + i;
-      if (row < nq) out[row * d_out + col] = acc[u][i] - bc;
+    for (int i = 0; i < 4; ++i)
+      T[(16 * wave + 4 * g + i) * kProjLds + 16 * u + r16] = acc[u][i] - bc[u];
+  __syncthreads();
+  const int64_t trow0 = (int64_t)blockIdx.x * 64;
+  const bool vec_out = (d_out & 3) == 0;
+  for (int e = t; e < 64 * 16; e += 256) {
+    const int rr = e >> 4, c4 = 4 * (e & 15);
+    const int64_t row = trow0 + rr;
+    const int col = col0 + c4;
+    if (row >= nq || col >= d_out) continue;
+    const float4 v = *reinterpret_cast<const float4*>(&T[rr * kProjLds + c4]);
+    float* o = out + row * d_out + col;
+    if (vec_out && col + 3 < d_out) {
+      *reinterpret_cast<float4*>(o) = v;
+    } else {
+      const float w[4] = {v.x, v.y, v.z, v.w};
+      for (int k = 0; k < 4 && col + k < d_out; ++k) o[k] = w[k];
     }
   }
 }
@@ -110,7 +145,11 @@ hipError_t launch_project(const float* Q, int64_t nq, int d, const float* R, int
   if (nq <= 0) return hipSuccess;
   const int64_t row_blocks = (nq + 63) / 64;
   dim3 grid((unsigned)row_blocks, (unsigned)((d_out + 63) / 64));
-  hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, st, Q, nq, d, R, d_out, bias, out);
+  if (d % 4 == 0 && (reinterpret_cast<uintptr_t>(Q) & 15) == 0)
+    hipLaunchKernelGGL(project_kernel<true>, grid, dim3(256), 0, st, Q, nq, d, R, d_out, bias, out);
+  else
+    hipLaunchKernelGGL(project_kernel<false>, grid, dim3(256), 0, st, Q, nq, d, R, d_out, bias,
+                       out);
   return hipGetLastError();
 }
 
