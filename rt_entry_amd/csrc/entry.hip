@@ -21,6 +21,24 @@
 
 namespace rte {
 
+// One step of the walk over node (a, b) = {lo[3], hi[3], skip, link}: a leaf
+// whose box holds the point ends the walk (hit = leaf id); an inner node
+// whose box holds it continues with its first child; otherwise the walk
+// continues at the skip link (later in preorder).
+__device__ __forceinline__ bool visit_node(uint4 a, uint4 b, float x, float y, float z, int& i,
+                                           int& hit) {
+  const bool in = (x >= __uint_as_float(a.x)) & (x <= __uint_as_float(a.w)) &
+                  (y >= __uint_as_float(a.y)) & (y <= __uint_as_float(b.x)) &
+                  (z >= __uint_as_float(a.z)) & (z <= __uint_as_float(b.y));
+  const int skip = (int)b.z, link = (int)b.w;
+  if (in && link >= 0) {
+    hit = link;
+    return true;
+  }
+  i = in ? -link - 1 : skip;
+  return false;
+}
+
 __global__ __launch_bounds__(256) void select_entry_kernel(const BoxNode* __restrict__ nodes,
                                                            int n_nodes,
                                                            const float* __restrict__ pq,
@@ -52,18 +70,19 @@ __global__ __launch_bounds__(256) void select_entry_kernel(const BoxNode* __rest
   const float z = pq[q * stride + 2];
   int hit = 0;
   int i = 0;
-  while (i < n_nodes) {
-    const BoxNode nd = i < n_top ? top[i] : nodes[i];
-    const bool in = x >= nd.lo[0] && x <= nd.hi[0] && y >= nd.lo[1] && y <= nd.hi[1] &&
-                    z >= nd.lo[2] && z <= nd.hi[2];
-    if (in) {
-      if (nd.link >= 0) {
-        hit = nd.link;
-        break;
-      }
-      i = -nd.link - 1;  // first child
-    } else {
-      i = nd.skip;  // later in preorder, so the walk ends
+  bool found = false;
+  // Two loops, one per address space: a node index that may point into LDS
+  // or global memory would make every read a generic (flat) load.  Each
+  // visit reads the whole 32-byte node as two 16-byte loads and tests the box
+  // without short-circuits, so a visit is one memory access.
+  const uint4* top4 = reinterpret_cast<const uint4*>(top);
+  const uint4* all4 = reinterpret_cast<const uint4*>(nodes);
+  while (!found && i < n_nodes) {
+    while (!found && i < n_top) {
+      found = visit_node(top4[2 * i], top4[2 * i + 1], x, y, z, i, hit);
+    }
+    while (!found && i >= n_top && i < n_nodes) {
+      found = visit_node(all4[2 * i], all4[2 * i + 1], x, y, z, i, hit);
     }
   }
   hits[q] = hit;
