@@ -515,26 +515,37 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
   const int m = W << shift;
   const int64_t np = a.np;
   const int32_t npi = (int32_t)np;
+  // the uniform shape as register values: with `n = a.beam` inside the
+  // loop, the per-query choice below became a select between the argument
+  // block's address and the array's, i.e. a generic (flat) load per query
+  const int beam_all = __builtin_amdgcn_readfirstlane(a.beam);
+  const int k_all = __builtin_amdgcn_readfirstlane(a.k);
 
   for (int64_t slot = next_query(a, -1, wave, lane); slot < a.nq;
        slot = next_query(a, slot, wave, lane)) {
     const int64_t q = query_of(a, slot);
     // per-query shape (uneven batches carry their own k and beam)
-    int n = a.beam, k = a.k;
+    int n = beam_all, k = k_all;
     if (a.beam_per_q) {
       n = a.beam_per_q[q];
       k = a.k_per_q[q];
-    }
-    int bwords = a.bloom_words, bhash = a.bloom_hashes;
-    if (a.bloom_auto) {  // auto_tune_bloom sizing per beam
-      const int lg = 31 - __clz(n);
-      bwords = a.auto_words[lg];
-      bhash = a.auto_hashes[lg];
     }
     // per-query shapes are wave-uniform: keep them in SGPRs so loops over
     // them are scalar branches, not exec-masked ones
     n = __builtin_amdgcn_readfirstlane(n);
     k = __builtin_amdgcn_readfirstlane(k);
+    int bwords = a.bloom_words, bhash = a.bloom_hashes;
+    if (a.bloom_auto) {  // auto_tune_bloom sizing per beam
+      // constant indices only: a computed index into the by-value argument
+      // block would turn these reads into generic (flat) loads
+      const int lg = 31 - __clz(n);
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+        if (t == lg) {
+          bwords = a.auto_words[t];
+          bhash = a.auto_hashes[t];
+        }
+    }
     bwords = __builtin_amdgcn_readfirstlane(bwords);
     bhash = __builtin_amdgcn_readfirstlane(bhash);
     WaveLds s;
