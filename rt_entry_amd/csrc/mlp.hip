@@ -341,17 +341,87 @@ __device__ __forceinline__ void soft_update(float& mx, float& se, int& ix, float
          __builtin_amdgcn_exp2f(__builtin_fmaf(z3, L2E, nl)));
 }
 
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) const void global_void_t;
+
+// Direct-to-LDS copy of one 64-column tile of W3^T (64 x K bf16) and its 64
+// biases.  The tile is stored in 16-byte units, unit(c, q) = c * CPR +
+// (q ^ (c & SWM)) for column c and k chunk q (8 bf16): the XOR spreads the
+// 16 columns that an MFMA operand read touches over distinct banks, without
+// padding.  Each global_load_lds_dwordx4 of a wave fills 64 consecutive
+// units; soff[i] is this lane's source offset for its i-th copy.  The copies
+// have no register destination, so they are issued where they stand (a
+// whole tile ahead of their use) and only the wait before the barrier at the
+// end of the tile covers them.
+template <int KS>
+__device__ __forceinline__ void head_dma_tile(const uint16_t* WT, const float* bias, int col0,
+                                              const int (&soff)[KS], uint4* dst, float* bdst,
+                                              int wave, int lane) {
+  constexpr int K = 32 * KS;
+#pragma unroll
+  for (int i = 0; i < KS; ++i)
+    __builtin_amdgcn_global_load_lds((global_void_t*)(WT + (int64_t)col0 * K + soff[i]),
+                                     (lds_void_t*)(dst + 64 * (4 * i + wave)), 16, 0, 0);
+  if (wave == 0)
+    __builtin_amdgcn_global_load_lds((global_void_t*)(bias + col0 + lane), (lds_void_t*)bdst,
+                                     4, 0, 0);
+}
+
+// Logits of one tile for the wave's 32 rows, fed into the online softmax:
+// the accumulators start from the bias; B fragments of k step s + 1 are read
+// while step s multiplies.
+template <int KS>
+__device__ __forceinline__ void head_tile(const uint4* Bt, const float* bt, int r, int g, int xr,
+                                          const uint4 (&A)[2][KS], int n0, float (&mx)[2][4],
+                                          float (&se)[2][4], int (&ix)[2][4]) {
+  constexpr int CPR = 4 * KS;
+  floatx4 acc[2][4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float bj = bt[16 * j + r];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) acc[rb][j] = floatx4{bj, bj, bj, bj};
+  }
+  uint4 bc[4], bn[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) bc[j] = Bt[(16 * j + r) * CPR + (g ^ xr)];
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    if (s + 1 < KS) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bn[j] = Bt[(16 * j + r) * CPR + ((4 * (s + 1) + g) ^ xr)];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) acc[rb][j] = mfma_k32(A[rb][s], bc[j], acc[rb][j]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bc[j] = bn[j];
+  }
+  // columns >= N carry a -inf bias (zero weights), so their logits are -inf
+  // and drop out of max and sum
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      soft_update(mx[rb][v], se[rb][v], ix[rb][v], acc[rb][0][v], acc[rb][1][v], acc[rb][2][v],
+                  acc[rb][3][v], n0 + r);
+}
+
 // Output layer fused with the softmax reduction, for 128 rows x one group of
 // `cg` columns per workgroup (grid.x = column group, so consecutive
 // workgroups, which land on different XCDs, read disjoint slices of W3).
 // Wave w keeps the bf16 H2 fragments of its 32 rows in registers for the
 // whole group; 64-column tiles of W3^T ([Npad][K] bf16, rows >= N zero,
-// Npad a multiple of 64) are double-buffered through LDS (one barrier per
-// tile).  The accumulators start from the bias, so each logit z = acc feeds
-// an online (max, lowest argmax column, sum exp) per row directly; logits
-// are never stored.  The 16 lanes sharing a row merge their states at the
-// end and write one SoftPart per (row, group).  K = 32 * KS; dynamic LDS
-// 2 * 64 * (K + 8) * 2 bytes; bias has Npad entries, -inf beyond N.
+// Npad a multiple of 64) and their biases are copied straight into two LDS
+// buffers, the next tile's copy in flight while the current tile is
+// multiplied (one wait and one barrier per tile; two separate buffers, so
+// the compiler can tell that the copy into one does not touch the reads of
+// the other).  The accumulators start from the bias, so each logit z = acc
+// feeds an online (max, lowest argmax column, sum exp) per row directly;
+// logits are never stored.  The 16 lanes sharing a row merge their states
+// at the end and write one SoftPart per (row, group).  K = 32 * KS; static
+// LDS 2 * 64 * K * 2 + 512 bytes; bias has Npad entries, -inf beyond N.
 template <int KS>
 __global__ __launch_bounds__(256, 2) void mlp_head_kernel(const uint16_t* __restrict__ H,
                                                           int64_t n,
@@ -359,16 +429,29 @@ __global__ __launch_bounds__(256, 2) void mlp_head_kernel(const uint16_t* __rest
                                                           const float* __restrict__ bias, int N,
                                                           int cg, SoftPart* __restrict__ parts) {
   constexpr int K = 32 * KS;
-  constexpr int LDB = K + 8;
-  constexpr int CPR = K / 8;  // 16-byte chunks per column
-  extern __shared__ __attribute__((aligned(16))) uint16_t Bs[];
+  constexpr int CPR = K / 8;  // 16-byte units per column
+  constexpr int SWM = (CPR < 16 ? CPR : 16) - 1;
+  __shared__ uint4 Bt0[64 * CPR];
+  __shared__ uint4 Bt1[64 * CPR];
+  __shared__ float bias0[64];
+  __shared__ float bias1[64];
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int r = lane & 15, g = lane >> 4;
+  const int xr = r & SWM;  // (16 j + r) & SWM for every column block j
   const int grp = blockIdx.x, G = gridDim.x;
   const int c0 = grp * cg;
   const int c1 = min(N, c0 + cg);
   const int64_t m0 = (int64_t)blockIdx.y * 128 + 32 * wave;
+  int soff[KS];  // element offset inside a tile of this lane's i-th copy
+#pragma unroll
+  for (int i = 0; i < KS; ++i) {
+    const int u = 64 * (4 * i + wave) + lane;
+    const int c = u / CPR;
+    soff[i] = c * K + 8 * ((u % CPR) ^ (c & SWM));
+  }
+  const int ntile = (c1 - c0 + 63) >> 6;
+  head_dma_tile<KS>(WT, bias, c0, soff, Bt0, bias0, wave, lane);
   uint4 A[2][KS];
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb) {
@@ -378,12 +461,6 @@ __global__ __launch_bounds__(256, 2) void mlp_head_kernel(const uint16_t* __rest
       A[rb][s] = row < n ? *reinterpret_cast<const uint4*>(H + row * K + 32 * s + 8 * g)
                          : make_uint4(0, 0, 0, 0);
   }
-  // tile loader: thread t moves the 16-byte chunks t + 256 i (i < KS) of a
-  // 64-column tile, chunk q = column q / CPR, k offset 8 (q % CPR)
-  uint4 pre[KS];
-  const uint16_t* wt_t = WT + (int64_t)(t / CPR) * K + 8 * (t % CPR);
-  uint16_t* bs_t = Bs + (t / CPR) * LDB + 8 * (t % CPR);
-  constexpr int COLS_PER_STEP = 256 / CPR;  // columns advanced per i
   float mx[2][4], se[2][4];
   int ix[2][4];
 #pragma unroll
@@ -394,72 +471,25 @@ __global__ __launch_bounds__(256, 2) void mlp_head_kernel(const uint16_t* __rest
       se[rb][v] = 0.f;
       ix[rb][v] = 0x7fffffff;
     }
-  const int ntile = (c1 - c0 + 63) >> 6;
-  // bias of the lane's four columns of the current and of the next tile
-  // (padded to whole tiles with -inf)
-  float bcur[4], bnext[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) bcur[j] = bias[c0 + 16 * j + r];
-#pragma unroll
-  for (int i = 0; i < KS; ++i)
-    pre[i] = *reinterpret_cast<const uint4*>(wt_t + (int64_t)(c0 + COLS_PER_STEP * i) * K);
-#pragma unroll
-  for (int i = 0; i < KS; ++i)
-    *reinterpret_cast<uint4*>(bs_t + COLS_PER_STEP * i * LDB) = pre[i];
+  __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): the first tile is in LDS
   __syncthreads();
-  for (int tile = 0; tile < ntile; ++tile) {
-    const int n0 = c0 + 64 * tile;
-    const int buf = tile & 1;
-    // next tile (the group's first again after the last: loaded, stored
-    // into the idle buffer and never read)
-    const int nn = tile + 1 < ntile ? n0 + 64 : c0;
-#pragma unroll
-    for (int i = 0; i < KS; ++i)
-      pre[i] = *reinterpret_cast<const uint4*>(wt_t + (int64_t)(nn + COLS_PER_STEP * i) * K);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) bnext[j] = bias[nn + 16 * j + r];
-    // (the compiler issues these loads during the softmax below; nothing
-    // waits on them before the LDS store at the end of the tile, and the
-    // accumulators start from the bias fetched one tile earlier)
-    floatx4 acc[2][4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb) acc[rb][j] = floatx4{bcur[j], bcur[j], bcur[j], bcur[j]};
-    {  // B fragments of k step s + 1 are read from LDS while step s multiplies
-      const uint16_t* bb = &Bs[(buf * 64 + r) * LDB + 8 * g];
-      uint4 bc[4], bn[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bc[j] = *reinterpret_cast<const uint4*>(bb + 16 * j * LDB);
-#pragma unroll
-      for (int s = 0; s < KS; ++s) {
-        if (s + 1 < KS) {
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            bn[j] = *reinterpret_cast<const uint4*>(bb + 16 * j * LDB + 32 * (s + 1));
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-          for (int rb = 0; rb < 2; ++rb) acc[rb][j] = mfma_k32(A[rb][s], bc[j], acc[rb][j]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) bc[j] = bn[j];
-      }
+  // two tiles per trip, so each half names its buffers statically; a tile
+  // past the group's last copies the group's first again (never read)
+  for (int tile = 0; tile < ntile; tile += 2) {
+    {
+      const int n0 = c0 + 64 * tile;
+      head_dma_tile<KS>(WT, bias, tile + 1 < ntile ? n0 + 64 : c0, soff, Bt1, bias1, wave, lane);
+      head_tile<KS>(Bt0, bias0, r, g, xr, A, n0, mx, se, ix);
+      __builtin_amdgcn_s_waitcnt(0x0f70);
+      __syncthreads();
     }
-    // columns >= N carry a -inf bias (zero weights), so their logits are
-    // -inf and drop out of max and sum
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        soft_update(mx[rb][v], se[rb][v], ix[rb][v], acc[rb][0][v], acc[rb][1][v], acc[rb][2][v],
-                    acc[rb][3][v], n0 + r);
-#pragma unroll
-    for (int i = 0; i < KS; ++i)
-      *reinterpret_cast<uint4*>(bs_t + ((buf ^ 1) * 64 + COLS_PER_STEP * i) * LDB) = pre[i];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) bcur[j] = bnext[j];
-    __syncthreads();
+    if (tile + 1 < ntile) {
+      const int n0 = c0 + 64 * (tile + 1);
+      head_dma_tile<KS>(WT, bias, tile + 2 < ntile ? n0 + 64 : c0, soff, Bt0, bias0, wave, lane);
+      head_tile<KS>(Bt1, bias1, r, g, xr, A, n0, mx, se, ix);
+      __builtin_amdgcn_s_waitcnt(0x0f70);
+      __syncthreads();
+    }
   }
   // merge the 16 lanes of each row (lanes differing in bits 0..3)
 #pragma unroll
@@ -1049,7 +1079,7 @@ int predict_impl(rte_ctx* c, const float* X, int64_t ldx, int64_t n, const int32
   const int cb = cbw <= 1 ? 1 : cbw <= 2 ? 2 : cbw <= 4 ? 4 : 8;
   const int ks = d2 / 32;
   const size_t lds_h = (size_t)32 * std::max((d0 + 8) + (d1 + 8), d2 + 8) * 2;
-  const size_t lds_o = (size_t)2 * 64 * (d2 + 8) * 2;
+  const size_t lds_o = 0;  // the head kernel's LDS is static
   const void* hk = cb == 1   ? (const void*)rte::mlp_hidden_kernel<1>
                    : cb == 2 ? (const void*)rte::mlp_hidden_kernel<2>
                    : cb == 4 ? (const void*)rte::mlp_hidden_kernel<4>
