@@ -847,7 +847,12 @@ int rte_autotune_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
   int rc = RTE_OK;
   for (int bi = 0; bi < n_bloom && rc == RTE_OK; ++bi)
     for (int wpb : {1, 2, 4})
-      for (int mult : {0, 1, 2}) {  // 0: one workgroup per wpb queries
+      // 0: one workgroup per wpb queries; 1, 2: the resident grid times
+      // mult; -1: the fewest waves that take the same number of rounds
+      // through the queue as the resident grid (every round full: 10k
+      // queries on 4096 resident waves run 3334 waves, 3 full rounds,
+      // instead of 4096 + 4096 + 1808)
+      for (int mult : {0, 1, 2, -1}) {
         rte_params t = *p;
         t.bloom_words = bloom_words[bi];
         t.wg_waves = wpb;
@@ -866,8 +871,17 @@ int rte_autotune_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
         probe.bloom_words_max = std::max(bw, 2048);
         int occ = 1;
         (void)rte::search_occupancy(probe, 256, wpb, &occ);
-        t.grid = mult == 0 ? (int)std::min<int64_t>((nq + wpb - 1) / wpb, 1 << 20)
-                           : std::max(1, occ) * cus * mult;
+        const int64_t resident = (int64_t)std::max(1, occ) * cus;  // workgroups
+        if (mult == 0) {
+          t.grid = (int)std::min<int64_t>((nq + wpb - 1) / wpb, 1 << 20);
+        } else if (mult > 0) {
+          t.grid = (int)(resident * mult);
+        } else {
+          const int64_t waves = resident * wpb;
+          const int64_t rounds = (nq + waves - 1) / waves;
+          const int64_t need = (nq + rounds - 1) / rounds;  // waves per round
+          t.grid = (int)std::max<int64_t>(1, (need + wpb - 1) / wpb);
+        }
         float ms_sum = 0.f;
         for (int rep = 0; rep < 4 && rc == RTE_OK; ++rep) {
           (void)hipEventRecord(e0, c->stream);
