@@ -67,3 +67,41 @@ def test_mlp_training_reduces_loss_and_predicts(ctx):
     # a confidence threshold above 1 sends every query to the fallback
     leaf2 = ctx.mlp_predict(Xd, fallback=fb, min_conf=1.5).cpu().numpy()
     assert (leaf2 == 7).all()
+
+
+@pytest.mark.parametrize("d2", [32, 128, 256, 512])
+def test_mlp_predict_every_head_width(ctx, d2):
+    """The inference kernels for every output-layer width they are built for
+    (K = 32 ... 512: one to sixteen 32-wide k steps, the widest with 128 KB of
+    LDS per workgroup) against the FP64 oracle with the same bf16 roundings;
+    1000 leaves (not a multiple of the column-group width) and enough rows
+    for several row blocks."""
+    import torch
+    from oracle import mlp as M
+    rng = np.random.default_rng(d2)
+    d0, d1, n_out, n = 64, 128, 1000, 3000
+    ctx.mlp_create(d0, d1, d2, n_out, seed=d2)
+    W, b = ctx.mlp_get()
+    W = [w * 2 for w in W]
+    b = [rng.normal(0, 0.3, size=x.shape).astype(np.float32) for x in b]
+    ctx.mlp_set(W, b)
+    mean = rng.normal(size=d0).astype(np.float32)
+    inv = rng.uniform(0.5, 1.5, size=d0).astype(np.float32)
+    ctx.mlp_set_input_norm(mean, inv)
+    X = rng.normal(size=(n, d0)).astype(np.float32)
+    fb = np.full(n, 3, np.int32)
+    conf = torch.empty(n, dtype=torch.float32, device="cuda")
+    print(f"[mlp] predict d2={d2}", flush=True)
+    leaf = ctx.mlp_predict(torch.from_numpy(X).cuda(), fallback=torch.from_numpy(fb).cuda(),
+                           min_conf=0.0, conf=conf).cpu().numpy()
+    ol, op = M.predict(W, b, X, fallback=fb, min_conf=0.0, norm=(mean, inv))
+    agree = (ol == leaf).mean()
+    cd = np.abs(conf.cpu().numpy() - op)
+    print(f"[mlp] d2={d2} agree {agree:.4f} conf diff median {np.median(cd):.2e} "
+          f"max {cd.max():.2e}", flush=True)
+    assert agree > 0.99, agree
+    # FP32 accumulation against FP64 leaves most confidences equal to ~1e-6;
+    # the largest differences come from hidden activations whose bf16
+    # rounding flips between the two (one bf16 ulp, 2^-8 relative)
+    assert np.median(cd) < 1e-4, np.median(cd)
+    assert cd.max() < 1e-2, cd.max()
