@@ -335,8 +335,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(queries shard across GPUs with no exchange, see the design notes)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 (queries "
+                          "would shard across GPUs with no exchange: each GPU holds the "
+                          "whole index and searches its share of the batch)"}))
         sys.exit(2)
     if args.workload == 3:  # the uneven-batch workload has its own measurement
         bench_uneven(args)
