@@ -15,11 +15,14 @@
 // Large nodes (more than kKdSmall points) are split a level at a time:
 //   box       one workgroup per 4096-point chunk of a node, min/max of
 //             order-preserving integer keys of the coordinates, atomics per node
-//   bisection every node of the level runs findxM's loop in lockstep: one
-//             counting kernel (chunks, one atomic per workgroup) and one
-//             per-node update kernel per step; the host stops the steps when
-//             no node is left (read back every fourth step)
-//   partition wave-aggregated slots from both ends of the node's segment
+//   bisection the split-axis coordinates of the level are gathered once into
+//             a contiguous array; every node of the level runs findxM's loop
+//             in lockstep: one counting kernel (chunks, one atomic per
+//             workgroup) and one per-node update kernel per step; the host
+//             stops the steps when no node is left (read back every fourth
+//             step)
+//   partition slots from both ends of the node's segment, one atomic per
+//             side and workgroup
 // Small nodes (at most kKdSmall points) are finished by one workgroup each:
 // ids, coordinates and a local permutation in LDS, an explicit stack of
 // segments, block-wide counts for the bisection.  Left children always come
@@ -201,20 +204,31 @@ __global__ void kd_split_init_kernel(LevelNode* nodes, int m) {
   n.done = !(br - bl > eps);
 }
 
-__global__ __launch_bounds__(256) void kd_count_kernel(const float* __restrict__ pts,
-                                                       int64_t stride,
-                                                       const int* __restrict__ idx,
+// vals[i] = the split-axis coordinate of point idx[i], once per level, so
+// the bisection steps and the partition read it contiguously
+__global__ __launch_bounds__(256) void kd_gather_kernel(const float* __restrict__ pts,
+                                                        int64_t stride,
+                                                        const int* __restrict__ idx,
+                                                        const LevelNode* nodes, int m,
+                                                        float* __restrict__ vals) {
+  const int nd = node_of_block(nodes, m, blockIdx.x);
+  const int axis = nodes[nd].axis;
+  const int64_t s0 = nodes[nd].l + ((int64_t)blockIdx.x - nodes[nd].blk0) * kKdChunk;
+  const int64_t s1 = min(nodes[nd].r, s0 + kKdChunk);
+  for (int64_t i = s0 + threadIdx.x; i < s1; i += 256)
+    vals[i] = pts[(int64_t)idx[i] * stride + axis];
+}
+
+__global__ __launch_bounds__(256) void kd_count_kernel(const float* __restrict__ vals,
                                                        LevelNode* nodes, int m) {
   __shared__ unsigned red[4];
   const int nd = node_of_block(nodes, m, blockIdx.x);
   if (nodes[nd].done) return;
   const float mid = kd_mid(nodes[nd].bl, nodes[nd].br);
-  const int axis = nodes[nd].axis;
   const int64_t s0 = nodes[nd].l + ((int64_t)blockIdx.x - nodes[nd].blk0) * kKdChunk;
   const int64_t s1 = min(nodes[nd].r, s0 + kKdChunk);
   unsigned c = 0;
-  for (int64_t i = s0 + threadIdx.x; i < s1; i += 256)
-    c += pts[(int64_t)idx[i] * stride + axis] <= mid;
+  for (int64_t i = s0 + threadIdx.x; i < s1; i += 256) c += vals[i] <= mid;
   c = block_sum(c, red);
   if (threadIdx.x == 0 && c) atomicAdd(&nodes[nd].cnt, c);
 }
@@ -240,33 +254,64 @@ __global__ void kd_update_kernel(LevelNode* nodes, int m, unsigned* act) {
   if (!n.done) atomicAdd(act, 1u);
 }
 
-__global__ __launch_bounds__(256) void kd_partition_kernel(const float* __restrict__ pts,
-                                                           int64_t stride,
+// Points of a chunk below the split go to the front of the node's segment,
+// the rest to the back.  Each thread holds its 16 points of the chunk; the
+// workgroup counts both sides, takes its slots with one atomic per side and
+// node, and the waves place their points from per-wave bases.
+__global__ __launch_bounds__(256) void kd_partition_kernel(const float* __restrict__ vals,
                                                            const int* __restrict__ idx,
                                                            int* __restrict__ idx2,
                                                            LevelNode* nodes, int m) {
+  constexpr int PER = kKdChunk / 256;
+  __shared__ unsigned cnt_w[2][4];
+  __shared__ unsigned base_w[2][4];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int nd = node_of_block(nodes, m, blockIdx.x);
   const int64_t l = nodes[nd].l, r = nodes[nd].r;
   const float xm = nodes[nd].xm;
-  const int axis = nodes[nd].axis;
   const int64_t s0 = l + ((int64_t)blockIdx.x - nodes[nd].blk0) * kKdChunk;
   const int64_t s1 = min(r, s0 + kKdChunk);
-  for (int64_t base = s0; base < s1; base += 256) {
-    const int64_t i = base + threadIdx.x;
+  int id[PER];
+  unsigned lmask = 0, vmask = 0;  // bit k: element k is valid / goes left
+  unsigned nl = 0, nr = 0;        // this wave's counts (uniform)
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int64_t i = s0 + 256 * k + t;
     const bool valid = i < s1;
-    const int id = valid ? idx[i] : 0;
-    const bool left = valid && pts[(int64_t)id * stride + axis] < xm;
-    const bool right = valid && !left;
-    const unsigned long long ml = __ballot(left), mr = __ballot(right);
-    unsigned lb = 0, rb = 0;
-    if ((threadIdx.x & 63) == 0) {
-      if (ml) lb = atomicAdd(&nodes[nd].nl, (unsigned)__popcll(ml));
-      if (mr) rb = atomicAdd(&nodes[nd].nr, (unsigned)__popcll(mr));
+    id[k] = valid ? idx[i] : 0;
+    const bool left = valid && vals[i] < xm;
+    vmask |= (unsigned)valid << k;
+    lmask |= (unsigned)left << k;
+    nl += (unsigned)__popcll(__ballot(left));
+    nr += (unsigned)__popcll(__ballot(valid && !left));
+  }
+  if (lane == 0) {
+    cnt_w[0][wave] = nl;
+    cnt_w[1][wave] = nr;
+  }
+  __syncthreads();
+  if (t == 0) {
+    const unsigned tl = cnt_w[0][0] + cnt_w[0][1] + cnt_w[0][2] + cnt_w[0][3];
+    const unsigned tr = cnt_w[1][0] + cnt_w[1][1] + cnt_w[1][2] + cnt_w[1][3];
+    unsigned bl = tl ? atomicAdd(&nodes[nd].nl, tl) : 0u;
+    unsigned br = tr ? atomicAdd(&nodes[nd].nr, tr) : 0u;
+    for (int w = 0; w < 4; ++w) {
+      base_w[0][w] = bl;
+      base_w[1][w] = br;
+      bl += cnt_w[0][w];
+      br += cnt_w[1][w];
     }
-    lb = (unsigned)__builtin_amdgcn_readfirstlane((int)lb);
-    rb = (unsigned)__builtin_amdgcn_readfirstlane((int)rb);
-    if (left) idx2[l + lb + lanes_below(ml)] = id;
-    if (right) idx2[r - 1 - (int64_t)(rb + lanes_below(mr))] = id;
+  }
+  __syncthreads();
+  unsigned ol = base_w[0][wave], orr = base_w[1][wave];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const bool valid = (vmask >> k) & 1u, left = (lmask >> k) & 1u;
+    const unsigned long long ml = __ballot(left), mr = __ballot(valid && !left);
+    if (left) idx2[l + ol + lanes_below(ml)] = id[k];
+    if (valid && !left) idx2[r - 1 - (int64_t)(orr + lanes_below(mr))] = id[k];
+    ol += (unsigned)__popcll(ml);
+    orr += (unsigned)__popcll(mr);
   }
 }
 
@@ -466,7 +511,7 @@ int rte_build_kdtree_gpu(rte_ctx* c, const float* pts, int64_t np, int64_t strid
   do {                                  \
     if (int r_ = ck((expr), what)) return r_; \
   } while (0)
-  DevBuf pin, idxb, idx2b, nodesb, actb, segb, leafb, ctrb, ordb, pidb, boxb;
+  DevBuf pin, idxb, idx2b, valsb, nodesb, actb, segb, leafb, ctrb, ordb, pidb;
   const float* dp = pts;
   int64_t ds = stride;
   if (where == RTE_HOST) {  // only the three coordinates travel
@@ -482,6 +527,8 @@ int rte_build_kdtree_gpu(rte_ctx* c, const float* pts, int64_t np, int64_t strid
   }
   KCK(idxb.reserve((size_t)np * 4), "hipMalloc");
   KCK(idx2b.reserve((size_t)np * 4), "hipMalloc");
+  KCK(valsb.reserve((size_t)np * 4), "hipMalloc");
+  float* vals = valsb.as<float>();
   KCK(actb.reserve(kKdMaxIter * sizeof(unsigned)), "hipMalloc");
   KCK(ctrb.reserve(2 * sizeof(unsigned)), "hipMalloc");
   int* idx = idxb.as<int>();
@@ -512,10 +559,12 @@ int rte_build_kdtree_gpu(rte_ctx* c, const float* pts, int64_t np, int64_t strid
     hipLaunchKernelGGL(kd_box_kernel, dim3((unsigned)blocks), dim3(256), 0, st, dp, ds, idx, nodes,
                        m);
     hipLaunchKernelGGL(kd_split_init_kernel, dim3(gm), dim3(256), 0, st, nodes, m);
+    hipLaunchKernelGGL(kd_gather_kernel, dim3((unsigned)blocks), dim3(256), 0, st, dp, ds, idx,
+                       nodes, m, vals);
     bool converged = false;
     for (int it = 0; it < kKdMaxIter; ++it) {
-      hipLaunchKernelGGL(kd_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, dp, ds, idx,
-                         nodes, m);
+      hipLaunchKernelGGL(kd_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, vals, nodes,
+                         m);
       hipLaunchKernelGGL(kd_update_kernel, dim3(gm), dim3(256), 0, st, nodes, m, act + it);
       if ((it & 3) == 3 || it == kKdMaxIter - 1) {
         unsigned left = 0;
@@ -528,7 +577,7 @@ int rte_build_kdtree_gpu(rte_ctx* c, const float* pts, int64_t np, int64_t strid
       }
     }
     if (!converged) return rte_fail(c, RTE_EBUILD, "kd-tree split search did not end");
-    hipLaunchKernelGGL(kd_partition_kernel, dim3((unsigned)blocks), dim3(256), 0, st, dp, ds, idx,
+    hipLaunchKernelGGL(kd_partition_kernel, dim3((unsigned)blocks), dim3(256), 0, st, vals, idx,
                        idx2, nodes, m);
     hipLaunchKernelGGL(kd_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, st, idx2, idx, nodes,
                        m);
