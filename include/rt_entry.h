@@ -148,7 +148,7 @@ int rte_set_rerank_base(rte_ctx* ctx, const float* base, int64_t np, int d, int6
 int rte_rerank(rte_ctx* ctx, const float* Q, int64_t nq, int64_t q_stride, const int32_t* cand,
                int n_cand, int k, int32_t* ids_out, float* dists_out, int where);
 /* PCA fit: FP64 mean and covariance / (n - 1) on the GPU (two passes,
- * fixed-order reductions), FP64 Jacobi eigensolver on the host; R_out d x d
+ * fixed-order reductions), FP64 Householder + QL eigensolver on the host; R_out d x d
  * (host, column j = eigenvector of the j-th largest eigenvalue, largest-
  * magnitude component positive), mu_out d, eig_out d (may be NULL).  X is
  * n x d with row stride `stride` (host or device), n >= 2, d <= 4096.

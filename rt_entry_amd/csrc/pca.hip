@@ -1,5 +1,5 @@
 // PCA fit on the GPU: FP64 mean and covariance of the base, symmetric
-// eigen-decomposition on the host (cyclic Jacobi, FP64), eigenvectors sorted
+// eigen-decomposition on the host (Householder + QL, FP64), eigenvectors sorted
 // by descending eigenvalue with a deterministic sign (largest-magnitude
 // component positive).  Replaces the reference's Eigen-based PCA::PCA
 // (pca.cpp:4-58: mean, covariance / (n - 1), SelfAdjointEigenSolver, columns
@@ -168,52 +168,151 @@ __global__ __launch_bounds__(256) void pca_rotate_kernel(const float* __restrict
   }
 }
 
-// Cyclic Jacobi eigen-decomposition of a symmetric d x d matrix (FP64):
-// A is destroyed, V gets the eigenvectors as columns, w the eigenvalues.
-// From the fifth sweep on, an off-diagonal element too small to change
-// either diagonal element is set to zero without a rotation, so the sweeps
-// end with an exactly diagonal matrix.
-static void jacobi_eigen(std::vector<double>& A, int d, std::vector<double>& V,
-                         std::vector<double>& w) {
-  V.assign((size_t)d * d, 0.0);
-  for (int i = 0; i < d; ++i) V[(size_t)i * d + i] = 1.0;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0;
-    for (int i = 0; i < d; ++i)
-      for (int j = i + 1; j < d; ++j) off += fabs(A[(size_t)i * d + j]);
-    if (off == 0.0) break;
-    for (int p = 0; p < d - 1; ++p)
-      for (int q = p + 1; q < d; ++q) {
-        const double apq = A[(size_t)p * d + q];
-        const double app = A[(size_t)p * d + p], aqq = A[(size_t)q * d + q];
-        const double g = 100.0 * fabs(apq);
-        if (sweep >= 4 && fabs(app) + g == fabs(app) && fabs(aqq) + g == fabs(aqq)) {
-          A[(size_t)p * d + q] = A[(size_t)q * d + p] = 0.0;
-          continue;
+// Symmetric eigen-decomposition (FP64) by Householder reduction to
+// tridiagonal form and the QL algorithm with implicit shifts (the EISPACK
+// tred2 / tql2 pair, as in Eigen's SelfAdjointEigenSolver that pca.cpp
+// uses), arranged so every inner loop walks rows: about 1.3 s at d = 960
+// and a few ms at d = 128.
+//
+// Householder step: A (d x d, row-major, only its lower triangle used) is
+// destroyed; on return A holds Q with A_in = Q T Q^T, diag the diagonal of T
+// and off[i] = T[i][i-1] (off[0] = 0).
+static void householder_tridiag(std::vector<double>& A, int d, std::vector<double>& diag,
+                                std::vector<double>& off) {
+  diag.assign(d, 0.0);
+  off.assign(d, 0.0);
+  auto a = [&](int i, int j) -> double& { return A[(size_t)i * d + j]; };
+  for (int i = d - 1; i > 0; --i) {
+    const int l = i - 1;
+    double h = 0.0;
+    if (l > 0) {
+      double scale = 0.0;
+      for (int k = 0; k <= l; ++k) scale += fabs(a(i, k));
+      if (scale == 0.0) {
+        off[i] = a(i, l);
+      } else {
+        for (int k = 0; k <= l; ++k) {
+          a(i, k) /= scale;
+          h += a(i, k) * a(i, k);
         }
-        if (apq == 0.0) continue;
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double c = 1.0 / sqrt(tt * tt + 1.0), s = tt * c;
-        for (int k = 0; k < d; ++k) {  // columns p, q
-          const double akp = A[(size_t)k * d + p], akq = A[(size_t)k * d + q];
-          A[(size_t)k * d + p] = c * akp - s * akq;
-          A[(size_t)k * d + q] = s * akp + c * akq;
+        const double f = a(i, l);
+        const double g = f >= 0.0 ? -sqrt(h) : sqrt(h);
+        off[i] = scale * g;
+        h -= f * g;
+        a(i, l) = f - g;
+        // p = A[0..l][0..l] u (u = row i), from the lower triangle row by row
+        const double* u = &A[(size_t)i * d];
+        for (int j = 0; j <= l; ++j) off[j] = 0.0;
+        for (int j = 0; j <= l; ++j) {
+          const double* aj = &A[(size_t)j * d];
+          double pj = aj[j] * u[j];
+          for (int k = 0; k < j; ++k) {
+            pj += aj[k] * u[k];
+            off[k] += aj[k] * u[j];
+          }
+          off[j] += pj;
         }
-        for (int k = 0; k < d; ++k) {  // rows p, q
-          const double apk = A[(size_t)p * d + k], aqk = A[(size_t)q * d + k];
-          A[(size_t)p * d + k] = c * apk - s * aqk;
-          A[(size_t)q * d + k] = s * apk + c * aqk;
+        double ff = 0.0;
+        for (int j = 0; j <= l; ++j) {
+          a(j, i) = u[j] / h;
+          off[j] /= h;
+          ff += off[j] * u[j];
         }
-        for (int k = 0; k < d; ++k) {
-          const double vkp = V[(size_t)k * d + p], vkq = V[(size_t)k * d + q];
-          V[(size_t)k * d + p] = c * vkp - s * vkq;
-          V[(size_t)k * d + q] = s * vkp + c * vkq;
+        const double hh = ff / (h + h);
+        for (int j = 0; j <= l; ++j) {  // A -= u q^T + q u^T on the lower triangle
+          const double fj = a(i, j);
+          const double gj = off[j] - hh * fj;
+          off[j] = gj;
+          for (int k = 0; k <= j; ++k) a(j, k) -= fj * off[k] + gj * a(i, k);
         }
       }
+    } else {
+      off[i] = a(i, l);
+    }
+    diag[i] = h;
   }
-  w.resize(d);
-  for (int i = 0; i < d; ++i) w[i] = A[(size_t)i * d + i];
+  diag[0] = 0.0;
+  off[0] = 0.0;
+  // accumulate the reflections into A, which becomes Q
+  std::vector<double> gv(d);
+  for (int i = 0; i < d; ++i) {
+    const int l = i - 1;
+    if (diag[i] != 0.0) {
+      // g = a(i, 0..l) A[0..l][0..l], then A[k][0..l] -= a(k, i) g, row by row
+      for (int j = 0; j <= l; ++j) gv[j] = 0.0;
+      for (int k = 0; k <= l; ++k) {
+        const double aik = a(i, k);
+        const double* ak = &A[(size_t)k * d];
+        for (int j = 0; j <= l; ++j) gv[j] += aik * ak[j];
+      }
+      for (int k = 0; k <= l; ++k) {
+        const double aki = a(k, i);
+        double* ak = &A[(size_t)k * d];
+        for (int j = 0; j <= l; ++j) ak[j] -= aki * gv[j];
+      }
+    }
+    diag[i] = a(i, i);
+    a(i, i) = 1.0;
+    for (int j = 0; j <= l; ++j) a(j, i) = a(i, j) = 0.0;
+  }
+}
+
+// QL with implicit shifts on the tridiagonal (diag, off), rotating the rows
+// of Zt (row i = vector i, Q^T on entry): on return diag holds the
+// eigenvalues and row i of Zt the eigenvector of diag[i].  False if an
+// eigenvalue needed more than 60 iterations.
+static bool tridiag_ql(std::vector<double>& diag, std::vector<double>& off,
+                       std::vector<double>& Zt, int d) {
+  for (int i = 1; i < d; ++i) off[i - 1] = off[i];
+  off[d - 1] = 0.0;
+  for (int l = 0; l < d; ++l) {
+    int iter = 0;
+    int m;
+    do {
+      for (m = l; m < d - 1; ++m) {  // first negligible off-diagonal element
+        const double dd = fabs(diag[m]) + fabs(diag[m + 1]);
+        if (fabs(off[m]) <= 1e-15 * dd) break;
+      }
+      if (m != l) {
+        if (iter++ == 60) return false;
+        double g = (diag[l + 1] - diag[l]) / (2.0 * off[l]);
+        double r = hypot(g, 1.0);
+        g = diag[m] - diag[l] + off[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+        double s = 1.0, c = 1.0, p = 0.0;
+        int i;
+        for (i = m - 1; i >= l; --i) {
+          double f = s * off[i];
+          const double b = c * off[i];
+          r = hypot(f, g);
+          off[i + 1] = r;
+          if (r == 0.0) {  // underflow: deflate and restart
+            diag[i + 1] -= p;
+            off[m] = 0.0;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = diag[i + 1] - p;
+          r = (diag[i] - g) * s + 2.0 * c * b;
+          p = s * r;
+          diag[i + 1] = g + p;
+          g = c * r - b;
+          double* zi = &Zt[(size_t)i * d];
+          double* zj = zi + d;
+          for (int k = 0; k < d; ++k) {
+            f = zj[k];
+            zj[k] = s * zi[k] + c * f;
+            zi[k] = c * zi[k] - s * f;
+          }
+        }
+        if (r == 0.0 && i >= l) continue;
+        diag[l] -= p;
+        off[l] = g;
+        off[m] = 0.0;
+      }
+    } while (m != l);
+  }
+  return true;
 }
 
 }  // namespace rte
@@ -271,19 +370,24 @@ int rte_pca_fit(rte_ctx* c, const float* X, int64_t n, int d, int64_t stride, fl
                  "copy"))
     return r;
   if (int r = ck(hipStreamSynchronize(st), "sync")) return r;
-  std::vector<double> V, w;
-  rte::jacobi_eigen(C, d, V, w);
+  std::vector<double> w, off;
+  rte::householder_tridiag(C, d, w, off);
+  std::vector<double> Zt(dd);  // Q^T: row i = column i of Q
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) Zt[(size_t)j * d + i] = C[(size_t)i * d + j];
+  if (!rte::tridiag_ql(w, off, Zt, d))
+    return rte_fail(c, RTE_EBUILD, "pca: eigensolver did not converge");
   std::vector<int> order(d);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
   for (int j = 0; j < d; ++j) {
-    const int src = order[j];
+    const double* v = &Zt[(size_t)order[j] * d];
     int arg = 0;  // deterministic sign: largest-magnitude component positive
     for (int i = 1; i < d; ++i)
-      if (fabs(V[(size_t)i * d + src]) > fabs(V[(size_t)arg * d + src])) arg = i;
-    const double sg = V[(size_t)arg * d + src] < 0 ? -1.0 : 1.0;
-    for (int i = 0; i < d; ++i) R_out[(size_t)i * d + j] = (float)(sg * V[(size_t)i * d + src]);
-    if (eig_out) eig_out[j] = w[src];
+      if (fabs(v[i]) > fabs(v[arg])) arg = i;
+    const double sg = v[arg] < 0 ? -1.0 : 1.0;
+    for (int i = 0; i < d; ++i) R_out[(size_t)i * d + j] = (float)(sg * v[i]);
+    if (eig_out) eig_out[j] = w[order[j]];
   }
   for (int j = 0; j < d; ++j) mu_out[j] = (float)mu[j];
   return RTE_OK;
