@@ -319,7 +319,9 @@ __device__ __forceinline__ void soft_merge(float& mx, float& se, int& ix, float 
 
 // Online softmax update of one row with four logits of a tile (columns
 // col0 + 16 j), without branches: running max, lowest argmax column and
-// sum exp(z - max), the exponentials as 2^(z log2(e) - max log2(e)).
+// sum exp(z - max), the exponentials as 2^((z - max) log2(e)).  The
+// difference is taken before the scaling so that the maximum's own term is
+// exactly 1 (the confidence 1 / sum is then at most 1).
 // The running max starts finite (-1e30, below every logit) with se = 0;
 // logits are finite or -inf (padding columns), which contribute nothing.
 __device__ __forceinline__ void soft_update(float& mx, float& se, int& ix, float z0, float z1,
@@ -332,13 +334,10 @@ __device__ __forceinline__ void soft_update(float& mx, float& se, int& ix, float
   asm volatile("" : "+v"(tj));  // keep the select below a v_cndmask, not a branch
   ix = tm > mx ? col0 + tj : ix;
   const float nm = fmaxf(mx, tm);
-  const float nl = -nm * L2E;
-  se *= __builtin_amdgcn_exp2f(__builtin_fmaf(mx, L2E, nl));
+  se *= __builtin_amdgcn_exp2f((mx - nm) * L2E);
   mx = nm;
-  se += (__builtin_amdgcn_exp2f(__builtin_fmaf(z0, L2E, nl)) +
-         __builtin_amdgcn_exp2f(__builtin_fmaf(z1, L2E, nl))) +
-        (__builtin_amdgcn_exp2f(__builtin_fmaf(z2, L2E, nl)) +
-         __builtin_amdgcn_exp2f(__builtin_fmaf(z3, L2E, nl)));
+  se += (__builtin_amdgcn_exp2f((z0 - nm) * L2E) + __builtin_amdgcn_exp2f((z1 - nm) * L2E)) +
+        (__builtin_amdgcn_exp2f((z2 - nm) * L2E) + __builtin_amdgcn_exp2f((z3 - nm) * L2E));
 }
 
 typedef __attribute__((address_space(3))) void lds_void_t;

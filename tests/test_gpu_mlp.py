@@ -11,7 +11,8 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
 
 
-@pytest.mark.parametrize("shape", [(40, 48, 56, 100), (128, 256, 256, 512)])
+@pytest.mark.parametrize("shape", [(40, 48, 56, 100), (128, 256, 256, 512),
+                                   (960, 256, 256, 4096), (128, 256, 256, 16384)])
 def test_mlp_gradients_match_fp64_oracle(ctx, shape):
     import torch
     from oracle import mlp as M
@@ -32,10 +33,13 @@ def test_mlp_gradients_match_fp64_oracle(ctx, shape):
     oW, ob, oloss = M.grads(W, b, X, y, norm=(mean, inv))
     print(f"[mlp] shape={shape} loss gpu={loss} oracle={oloss}", flush=True)
     assert abs(loss - oloss) < 1e-4 * max(1.0, abs(oloss))
+    # FP32 accumulation over K terms against exact FP64 sums; the backward's
+    # widest reduction runs over the n_out output columns, and its error (with
+    # the bf16 roundings that error flips) grows with sqrt(n_out)
+    tol = 2e-4 * max(1.0, shape[3] / 4096) ** 0.5
     for l in range(3):
-        # FP32 accumulation over K terms against exact FP64 sums
-        assert _rel(gW[l], oW[l]) < 2e-4, (l, _rel(gW[l], oW[l]))
-        assert _rel(gb[l], ob[l]) < 2e-4, (l, _rel(gb[l], ob[l]))
+        assert _rel(gW[l], oW[l]) < tol, (l, _rel(gW[l], oW[l]))
+        assert _rel(gb[l], ob[l]) < tol, (l, _rel(gb[l], ob[l]))
 
 
 def test_mlp_training_reduces_loss_and_predicts(ctx):
@@ -69,17 +73,20 @@ def test_mlp_training_reduces_loss_and_predicts(ctx):
     assert (leaf2 == 7).all()
 
 
-@pytest.mark.parametrize("d2", [32, 128, 256, 512])
-def test_mlp_predict_every_head_width(ctx, d2):
+@pytest.mark.parametrize("d0,d1,d2,n_out", [(64, 128, 32, 1000), (64, 128, 128, 1000),
+                                             (64, 128, 256, 1000), (64, 128, 512, 1000),
+                                             (960, 256, 256, 4096), (128, 256, 256, 16384)])
+def test_mlp_predict_every_head_width(ctx, d0, d1, d2, n_out):
     """The inference kernels for every output-layer width they are built for
     (K = 32 ... 512: one to sixteen 32-wide k steps, the widest with 128 KB of
-    LDS per workgroup) against the FP64 oracle with the same bf16 roundings;
-    1000 leaves (not a multiple of the column-group width) and enough rows
-    for several row blocks."""
+    LDS per workgroup) and for the two network shapes of the learned-entry
+    workloads (960 inputs; 16384 leaves) against the FP64 oracle with the
+    same bf16 roundings; 1000 leaves (not a multiple of the column-group
+    width) and enough rows for several row blocks."""
     import torch
     from oracle import mlp as M
-    rng = np.random.default_rng(d2)
-    d0, d1, n_out, n = 64, 128, 1000, 3000
+    rng = np.random.default_rng(d2 if n_out == 1000 else d0 + n_out)
+    n = 3000
     ctx.mlp_create(d0, d1, d2, n_out, seed=d2)
     W, b = ctx.mlp_get()
     W = [w * 2 for w in W]
@@ -91,17 +98,19 @@ def test_mlp_predict_every_head_width(ctx, d2):
     X = rng.normal(size=(n, d0)).astype(np.float32)
     fb = np.full(n, 3, np.int32)
     conf = torch.empty(n, dtype=torch.float32, device="cuda")
-    print(f"[mlp] predict d2={d2}", flush=True)
+    print(f"[mlp] predict {d0}-{d1}-{d2}-{n_out}", flush=True)
     leaf = ctx.mlp_predict(torch.from_numpy(X).cuda(), fallback=torch.from_numpy(fb).cuda(),
                            min_conf=0.0, conf=conf).cpu().numpy()
     ol, op = M.predict(W, b, X, fallback=fb, min_conf=0.0, norm=(mean, inv))
     agree = (ol == leaf).mean()
     cd = np.abs(conf.cpu().numpy() - op)
-    print(f"[mlp] d2={d2} agree {agree:.4f} conf diff median {np.median(cd):.2e} "
+    print(f"[mlp] {d0}-{d1}-{d2}-{n_out} agree {agree:.4f} conf diff median {np.median(cd):.2e} "
           f"max {cd.max():.2e}", flush=True)
     assert agree > 0.99, agree
     # FP32 accumulation against FP64 leaves most confidences equal to ~1e-6;
-    # the largest differences come from hidden activations whose bf16
-    # rounding flips between the two (one bf16 ulp, 2^-8 relative)
+    # the largest differences come from the few rows where a hidden
+    # activation's bf16 rounding flips between the two (one bf16 ulp, 2^-8
+    # relative), more often with 960 inputs
     assert np.median(cd) < 1e-4, np.median(cd)
-    assert cd.max() < 1e-2, cd.max()
+    assert np.quantile(cd, 0.99) < 1e-3, np.quantile(cd, 0.99)
+    assert cd.max() < 2e-2, cd.max()
