@@ -31,6 +31,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "QPS at recall@10≥0.95, 1M×128-d, learned vs kd-tree entry on 1 MI355X"
 HBM_PEAK = 8.0e12
+WORKLOAD_NAMES = {1: "Gaussian mixture (64 centres)", 2: "integer-valued mixture (1000 centres)",
+                  4: "low-rank mixture (500 components, rank 64, noise 0.05)",
+                  5: "integer-valued mixture (10k centres)"}
 
 
 def log(*a):
@@ -328,6 +331,11 @@ def main():
     ap.add_argument("--n-train", type=int, default=200_000, help="0 skips the learned entry")
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--input-norm", choices=("feature", "global"), default="feature",
+                    help="network input scaling: per-feature standard deviation, or one "
+                         "scale for all PCA columns (keeps their relative variances)")
+    ap.add_argument("--labels", default="",
+                    help="training labels file (.npz): loaded if it exists, else written")
     ap.add_argument("--min-conf", type=float, default=None,
                     help="kd-tree fallback below this confidence (default: fitted on "
                          "held-out queries from 0.1, 0.2, 0.3, 0.5)")
@@ -464,8 +472,18 @@ def main():
         # the plain search from every leaf), each a training pair: softer
         # targets than the single best leaf, which is often a near tie
         n_lab = 5
-        labm, _ = P.make_labels(ctx, PQt, None, gt_t, beam=64, mode=args.mode,
-                                kd=kd_h, n_leaf=W["n_leaf"], top=n_lab)
+        if args.labels and os.path.exists(args.labels):
+            with np.load(args.labels) as z:
+                labm = z["labels"]
+            if labm.shape != (PQt.shape[0], n_lab):
+                raise SystemExit(f"{args.labels}: labels {labm.shape}, expected "
+                                 f"{(PQt.shape[0], n_lab)}")
+            log(f"labels loaded from {args.labels}")
+        else:
+            labm, _ = P.make_labels(ctx, PQt, None, gt_t, beam=64, mode=args.mode,
+                                    kd=kd_h, n_leaf=W["n_leaf"], top=n_lab, log=log)
+            if args.labels:
+                np.savez(args.labels, labels=labm)
         labels = labm[:, 0]
         t_lab = time.time() - t0
         log(f"labels in {t_lab:.1f}s; {(labels != kd_h).mean():.3f} differ from the kd leaf")
@@ -489,6 +507,7 @@ def main():
         # kd-tree leaf
         losses, hist = P.train(ctx, X3, y3, W["n_leaf"],
                                epochs=args.epochs, batch=1024, lr=args.lr,
+                               input_norm=args.input_norm,
                                validate=lambda: validate(0.0),
                                log=(lambda m: log("train " + m)) if args.verbose else None)
         torch.cuda.synchronize()
@@ -508,6 +527,7 @@ def main():
             log(f"fallback threshold {min_conf[0]} ({mc_hist})")
         args.min_conf = min_conf[0]
         pred_info = {"train_queries": int(PQt.shape[0]), "labels_per_query": n_lab,
+                     "input_norm": args.input_norm,
                      "validation_queries": n_val,
                      "epochs_run": args.epochs, "best_epoch": best_ep[0],
                      "validation_recall@10_beam128_it32_no_fallback": round(best_ep[1], 4),
@@ -687,7 +707,8 @@ def main():
                 "launch": r["launch"]}
 
     out = {
-        "metric": METRIC,
+        "metric": METRIC if args.workload == 2 and args.scale == 1.0 else
+        METRIC.replace("1M×128-d", f"{W['n']}×{W['d']}-d (workload {args.workload})"),
         "value": round(head["qps"], 1),
         "unit": "queries/s",
         "n_gpus": 1,
@@ -700,7 +721,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{W['n']}x{W['d']} integer-valued mixture (workload {args.workload}), "
+            "workload": f"{W['n']}x{W['d']} {WORKLOAD_NAMES[args.workload]} "
+                        f"(workload {args.workload}), "
                         f"{nq} queries, degree-{W['cfg']['degree']} kNN+reverse graph, "
                         f"{W['n_leaf']} kd-tree leaves, k={k}",
             "entry": ("learned (kd-tree fallback), batched launch" if head is ln

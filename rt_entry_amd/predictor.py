@@ -48,13 +48,14 @@ def candidate_leaves(kd_hits: np.ndarray, gt: np.ndarray, belong: np.ndarray, k:
 
 def make_labels(ctx: Context, PQ, cand, gt, beam: int = 64, width: int = 1,
                 budgets=(4, 8, 16), mode: int = 0, kd=None, n_leaf: int | None = None,
-                rows_per_call: int = 1 << 22, top: int = 1):
+                rows_per_call: int = 1 << 22, top: int = 1, log=None):
     """Label per query (device search, device scoring).
 
     cand: (nq, C) int32 shortlist, -1 padded, column 0 the kd-tree leaf; or
     None to score all n_leaf leaves (kd = the kd-tree leaves for tie-breaks).
     Returns (labels int32[nq], best score int32[nq]); with top > 1 (all-leaf
-    scoring only) labels is (nq, top), the best `top` leaves in key order."""
+    scoring only) labels is (nq, top), the best `top` leaves in key order.
+    log: optional callable for progress lines (all-leaf scoring)."""
     import torch
     dev = PQ.device
     nq = PQ.shape[0]
@@ -88,6 +89,8 @@ def make_labels(ctx: Context, PQ, cand, gt, beam: int = 64, width: int = 1,
                 labels[q0:q0 + qn] = arg.int().cpu().numpy()
             best[q0:q0 + qn] = hits.gather(1, arg[:, None])[:, 0].int().cpu().numpy()
             del X, g, hits, key
+            if log is not None and (q0 // qc) % 16 == 15:
+                log(f"labels {q0 + qn}/{nq} queries")
         return labels, best
     C = cand.shape[1]
     qi_np, ci_np = np.nonzero(cand >= 0)
@@ -114,10 +117,14 @@ def make_labels(ctx: Context, PQ, cand, gt, beam: int = 64, width: int = 1,
 
 def train(ctx: Context, X, y, n_out: int, hidden=(256, 256), epochs: int = 20,
           batch: int = 1024, lr: float = 2e-3, seed: int = 1, log=None, validate=None,
-          every: int = 2):
+          every: int = 2, input_norm: str = "feature"):
     """Trains the MLP on device tensors X (n x d) and labels y (n).  The
-    input is standardised per feature with the training set's mean and
-    standard deviation (kept in the network).
+    input is centred with the training set's mean and scaled (kept in the
+    network): input_norm "feature" divides every feature by its own standard
+    deviation; "global" divides all features by one scale, the root mean
+    variance, which keeps the relative variances (for PCA coordinates whose
+    trailing columns are mostly noise, per-feature scaling would lift that
+    noise to the weight of the leading columns).
 
     validate: optional callable returning a score (higher is better) of the
     current network, e.g. end-to-end recall with the kd-tree fallback on held
@@ -129,7 +136,12 @@ def train(ctx: Context, X, y, n_out: int, hidden=(256, 256), epochs: int = 20,
     ctx.mlp_create(d, hidden[0], hidden[1], n_out, seed=seed)
     Xh = X.double()
     mean = Xh.mean(0)
-    std = Xh.std(0).clamp_min(1e-6)
+    if input_norm == "feature":
+        std = Xh.std(0).clamp_min(1e-6)
+    elif input_norm == "global":
+        std = Xh.var(0).mean().sqrt().clamp_min(1e-6).repeat(d)
+    else:
+        raise ValueError(f"input_norm {input_norm!r}")
     ctx.mlp_set_input_norm(mean.float().cpu().numpy(), (1.0 / std).float().cpu().numpy())
     g = torch.Generator(device="cpu")
     g.manual_seed(seed)
