@@ -175,6 +175,13 @@ int rte_pca_rotate(rte_ctx* ctx, const float* X, int64_t n, int d, int64_t strid
 int rte_knn(rte_ctx* ctx, const float* X, int64_t np, int d, int64_t stride, int k,
             int32_t* ids_out, float* dists_out, int where);
 
+/* rte_knn for the rows [row0, row0 + nrows) of X only, still against all np
+ * rows: ids_out/dists_out nrows x k.  A large build can run in pieces
+ * (progress between calls, bounded output staging) or split its row ranges
+ * across devices that each hold X.  No reference counterpart (as rte_knn). */
+int rte_knn_rows(rte_ctx* ctx, const float* X, int64_t np, int d, int64_t stride, int64_t row0,
+                 int64_t nrows, int k, int32_t* ids_out, float* dists_out, int where);
+
 /* Search graph from kNN lists (np x kf, each row by (distance, id), e.g.
  * from rte_knn): candidate edges of u are u -> knn[u][j] (rank j) and, for
  * every v with knn[v][j] == u and j < rev_rank, u -> v (rank j); a pair keeps

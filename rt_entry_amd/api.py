@@ -376,21 +376,27 @@ class Context:
         return aabbs[: nl.value].clone() if w == RTE_DEVICE else aabbs[: nl.value].copy(), \
             pids[: nl.value].clone() if w == RTE_DEVICE else pids[: nl.value].copy()
 
-    def knn(self, X, k: int, with_dists: bool = False):
+    def knn(self, X, k: int, with_dists: bool = False, rows=None):
         """Exact k nearest rows of X to each row of X (itself excluded), by
-        (distance, id); X host array or device tensor."""
+        (distance, id); X host array or device tensor.  rows = (row0, nrows)
+        restricts the output to those rows (still searched against all of X)."""
         px, w, keep = _ptr(X, np.float32)
         n, d = X.shape
+        row0, nr = (0, n) if rows is None else (int(rows[0]), int(rows[1]))
         if w == RTE_DEVICE:
             import torch
-            ids = torch.empty((n, k), dtype=torch.int32, device=X.device)
-            dists = torch.empty((n, k), dtype=torch.float32, device=X.device) if with_dists else None
+            ids = torch.empty((nr, k), dtype=torch.int32, device=X.device)
+            dists = torch.empty((nr, k), dtype=torch.float32, device=X.device) if with_dists else None
         else:
-            ids = np.empty((n, k), np.int32)
-            dists = np.empty((n, k), np.float32) if with_dists else None
+            ids = np.empty((nr, k), np.int32)
+            dists = np.empty((nr, k), np.float32) if with_dists else None
         pi, _, _ = _ptr(ids, np.int32)
         pd = _ptr(dists, np.float32)[0] if dists is not None else None
-        self._chk(lib().rte_knn(self._h, px, int(n), int(d), int(d), int(k), pi, pd, w))
+        if rows is None:
+            self._chk(lib().rte_knn(self._h, px, int(n), int(d), int(d), int(k), pi, pd, w))
+        else:
+            self._chk(lib().rte_knn_rows(self._h, px, int(n), int(d), int(d), row0, nr, int(k),
+                                         pi, pd, w))
         return (ids, dists) if with_dists else ids
 
     def graph_from_knn(self, knn, degree: int, rev_rank: int | None = None):

@@ -298,12 +298,15 @@ int rte_rerank(rte_ctx* c, const float* Q, int64_t nq, int64_t q_stride, const i
   return RTE_OK;
 }
 
-int rte_knn(rte_ctx* c, const float* X, int64_t np, int d, int64_t stride, int k,
-            int32_t* ids_out, float* dists_out, int where) {
+int rte_knn_rows(rte_ctx* c, const float* X, int64_t np, int d, int64_t stride, int64_t row0,
+                 int64_t nrows, int k, int32_t* ids_out, float* dists_out, int where) {
   if (!c) return RTE_EINVAL;
   if (!X || !ids_out || np < 2 || np >= 0x7fffffff || stride < d || stride % 4 != 0)
     return fail(c, RTE_EINVAL, "bad knn arguments (np %lld, d %d, stride %lld)", (long long)np, d,
                 (long long)stride);
+  if (row0 < 0 || nrows < 1 || row0 + nrows > np)
+    return fail(c, RTE_EINVAL, "knn rows [%lld, %lld) outside [0, %lld)", (long long)row0,
+                (long long)(row0 + nrows), (long long)np);
   if (d != 32 && d != 64 && d != 96 && d != 128)
     return fail(c, RTE_EUNSUP, "knn kernel compiled for d in {32, 64, 96, 128}, not %d", d);
   if (k < 1 || k > rte::kKnnK || k >= np)
@@ -315,25 +318,30 @@ int rte_knn(rte_ctx* c, const float* X, int64_t np, int d, int64_t stride, int k
     return fail(c, RTE_EINVAL, "X not 16-byte aligned");
   int32_t* di;
   float* dd = nullptr;
-  if (int r = stage_out(c, c->s_out, ids_out, (size_t)np * k, where, &di)) return r;
+  if (int r = stage_out(c, c->s_out, ids_out, (size_t)nrows * k, where, &di)) return r;
   if (dists_out)
-    if (int r = stage_out(c, c->s_aux2, dists_out, (size_t)np * k, where, &dd)) return r;
+    if (int r = stage_out(c, c->s_aux2, dists_out, (size_t)nrows * k, where, &dd)) return r;
   DevBuf norms;  // freed on return (hipFree waits for the device)
   HIPCHK(c, norms.reserve((size_t)np * sizeof(float)), "hipMalloc(norms)");
   HIPCHK(c, rte::launch_row_norms(dx, np, d, stride, norms.as<float>(), c->stream), "norms");
   // rows in launches of 128k (short kernels)
   const int64_t step = 1 << 17;
-  for (int64_t r0 = 0; r0 < np; r0 += step) {
-    const int64_t n = std::min<int64_t>(step, np - r0);
-    HIPCHK(c, rte::launch_knn(dx, np, d, stride, norms.as<float>(), r0, n, k, di + r0 * k,
-                              dd ? dd + r0 * k : nullptr, c->stream),
+  for (int64_t r0 = row0; r0 < row0 + nrows; r0 += step) {
+    const int64_t n = std::min<int64_t>(step, row0 + nrows - r0);
+    HIPCHK(c, rte::launch_knn(dx, np, d, stride, norms.as<float>(), r0, n, k, di + (r0 - row0) * k,
+                              dd ? dd + (r0 - row0) * k : nullptr, c->stream),
            "knn launch");
   }
-  if (int r = finish_out(c, ids_out, di, (size_t)np * k, where)) return r;
+  if (int r = finish_out(c, ids_out, di, (size_t)nrows * k, where)) return r;
   if (dists_out)
-    if (int r = finish_out(c, dists_out, dd, (size_t)np * k, where)) return r;
+    if (int r = finish_out(c, dists_out, dd, (size_t)nrows * k, where)) return r;
   HIPCHK(c, hipStreamSynchronize(c->stream), "sync knn");  // norms is freed on return
   return RTE_OK;
+}
+
+int rte_knn(rte_ctx* c, const float* X, int64_t np, int d, int64_t stride, int k,
+            int32_t* ids_out, float* dists_out, int where) {
+  return rte_knn_rows(c, X, np, d, stride, 0, np, k, ids_out, dists_out, where);
 }
 
 int rte_graph_from_knn(rte_ctx* c, const int32_t* knn, int64_t np, int kf, int degree,

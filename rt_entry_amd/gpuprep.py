@@ -58,6 +58,8 @@ def knn_gpu(X: torch.Tensor, Q: torch.Tensor | None, k: int, exclude_self: bool 
     if self_mode:
         Q = X
     dt = torch.float64 if fp64 else torch.float32
+    # at most ~8 GiB of distances per block of rows
+    block = max(64, min(block, (8 << 30) // (X.shape[0] * (8 if fp64 else 4))))
     Xd = X.to(dt)
     xn = (Xd * Xd).sum(1)
     nq = Q.shape[0]

@@ -25,6 +25,25 @@ def test_knn_exact_on_integer_data(ctx, n, d, k):
     np.testing.assert_array_equal(dists, ref_d.astype(np.float32))
 
 
+def test_knn_rows_equal_full(ctx):
+    """rte_knn_rows over row ranges gives exactly the matching rows of
+    rte_knn, for host and device inputs; ranges outside X are errors."""
+    import torch
+    from rt_entry_amd import RteError
+    X = _int_data(5000, 128, seed=4)
+    ids, dists = ctx.knn(X, 32, with_dists=True)
+    Xd = torch.from_numpy(X).cuda()
+    for r0, nr in ((0, 1), (1234, 3000), (4999, 1), (0, 5000)):
+        ri, rd = ctx.knn(X, 32, with_dists=True, rows=(r0, nr))
+        np.testing.assert_array_equal(ri, ids[r0:r0 + nr])
+        np.testing.assert_array_equal(rd, dists[r0:r0 + nr])
+        gi = ctx.knn(Xd, 32, rows=(r0, nr)).cpu().numpy()
+        np.testing.assert_array_equal(gi, ids[r0:r0 + nr])
+    for bad in ((4990, 20), (-1, 5), (10, 0)):
+        with pytest.raises(RteError):
+            ctx.knn(X, 32, rows=bad)
+
+
 def test_knn_float_data_agrees(ctx):
     import torch
     from rt_entry_amd import data as D
