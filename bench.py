@@ -304,6 +304,8 @@ def bench_uneven(args):
                          f"queries, degree-{W['cfg']['degree']} graph, {W['n_leaf']} kd-tree "
                          f"leaves, kd-tree entry, max_iter {max_iter}"),
             "launch": f"batched wg_waves={tuned.wg_waves} grid={tuned.grid}",
+            "seeds": {what: D.seed(3, i) for i, what in
+                      enumerate(("centres", "base", "queries", "train", "k_beam"))},
             "classes": len(cls), "search_kernel_ms": round(search_ms, 4),
             "per_class_launches": {"ms_per_step": round(t_c * 1e3, 4),
                                    "qps": round(nq / t_c, 1)},
@@ -375,6 +377,7 @@ def main():
 
     import oracle as O
     from rt_entry_amd import Context, make_params
+    from rt_entry_amd import data as D
     from rt_entry_amd import predictor as P
 
     dev = torch.device("cuda:0")
@@ -410,7 +413,6 @@ def main():
     ids = torch.empty((nq, k_out), dtype=torch.int32, device=dev)
     dists = torch.empty((nq, k_out), dtype=torch.float32, device=dev)
     if mixed:
-        from rt_entry_amd import data as D
         kq_h, _ = D.k_beam_assignment(5, nq, ks=(10, 100), beams=(128,))
         kq_d = torch.from_numpy(kq_h).to(dev)
         bq_d = {}
@@ -666,7 +668,8 @@ def main():
                     search_ms=search_ms, alg_bytes=alg, p=p, entry=entry, beam=best[1],
                     max_iter=best[2], iters=float(st[:, 2].mean()), pred_ms=pred_ms,
                     launch=(f"batched wg_waves={tuned.wg_waves} grid={tuned.grid}" if batched
-                            else "batched, default shape" if mixed else "grid"))
+                            else "batched, default shape" if mixed else "grid"),
+                    tuned=tuned, batched=batched, use=use)
 
     kd = kd_b = None
     if best_kd is not None:
@@ -679,6 +682,48 @@ def main():
         ln = measure(True, best_ln, batched=True)
         log(f"learned entry, batched launch: {ln['qps']:.0f} QPS recall {ln['recall']:.4f}")
     head = ln if (ln is not None and (ln["recall"] >= args.target or kd is None)) else kd
+
+    def measure_host_io(r):
+        """The headline step with the queries in pinned host memory and the
+        results copied back to pinned host memory, every copy on the query
+        stream: the PCIe-inclusive rate (reported beside the headline, which
+        starts and ends in HBM)."""
+        use_mlp[0] = r["use"]
+        Qh = W["Q"].cpu().pin_memory()
+        ids_h = torch.empty(ids.shape, dtype=ids.dtype).pin_memory()
+        dists_h = torch.empty(dists.shape, dtype=dists.dtype).pin_memory()
+        Qd = torch.empty_like(W["Q"])
+
+        def host_step():
+            Qd.copy_(Qh, non_blocking=True)
+            ctx.project(Qd, out=pq)
+            entry = entries(r["use"])
+            if r["batched"]:
+                ctx.search_batched(pq, entry, None, None, r["tuned"], k_stride=r["tuned"].k,
+                                   out=(ids, dists))
+            else:
+                search(entry, r["p"])
+            ids_h.copy_(ids, non_blocking=True)
+            dists_h.copy_(dists, non_blocking=True)
+
+        for _ in range(args.warmup):
+            host_step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            host_step()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        assert np.array_equal(ids_h.numpy(), ids.cpu().numpy())
+        return {"qps": round(nq * args.steps / wall, 1),
+                "ms_per_step": round(wall / args.steps * 1e3, 4),
+                "h2d_bytes_per_step": int(Qh.numel() * 4),
+                "d2h_bytes_per_step": int(ids_h.numel() * 4 + dists_h.numel() * 4),
+                "memory": "pinned host buffers, copies on the query stream"}
+
+    host_io = measure_host_io(head) if not (sd or mixed) else None
+    if host_io is not None:
+        log(f"host buffers in and out (PCIe included): {host_io['qps']:.0f} QPS")
 
     if args.copy_mib:
         # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters give
@@ -784,6 +829,9 @@ def main():
             "search_dims": d_search, "rerank": bool(sd),
             "graph_build": W["graph_build"],
             "prep_seconds": W["prep_seconds"],
+            "seeds": {what: D.seed(args.workload, i) for i, what in
+                      enumerate(("centres", "base", "queries", "train", "k_beam"))},
+            "pcie_inclusive": host_io,
             "kd_tree_entry": summary(kd),
             "kd_tree_entry_batched": summary(kd_b),
             "learned_entry_batched": summary(ln),
