@@ -395,14 +395,16 @@ def main():
     ctx.set_base(W["base"])
     ctx.set_graph(W["graph"])
     ctx.set_leaves(W["aabbs"], W["pids"])
-    ctx.set_pca(W["R"], W["mu"], W["d"])
     sd = args.search_dims if 0 < args.search_dims < W["d"] else 0
     d_search = sd or W["d"]
+    # the projection keeps the columns the search reads (graph.cu:262 projects
+    # onto the first DIM columns); the entry network takes the same input
+    ctx.set_pca(W["R"], W["mu"], d_search)
     if sd:  # the graph walk reads the first sd PCA columns (row stride d)
         ctx.set_base(W["base"], d=sd)
         ctx.set_rerank_base(W["raw"])
 
-    pq = torch.empty((nq, W["d"]), dtype=torch.float32, device=dev)
+    pq = torch.empty((nq, d_search), dtype=torch.float32, device=dev)
     hits = torch.empty(nq, dtype=torch.int32, device=dev)
     learned = torch.empty(nq, dtype=torch.int32, device=dev)
     # the combined large-scale workload: per-query k in {10, 100} (one beam

@@ -24,7 +24,77 @@ __device__ __forceinline__ bool key_less(float da, int ia, float db, int ib) {
   return da < db || (da == db && ia < ib);
 }
 
+// LDS staging of the candidate rows: 64 dims of the 64 candidates of a
+// round at a time, rows padded by 4 floats (a lane's 16-byte reads of its
+// own row then spread over the banks), followed by the same 64 query dims
+constexpr int kRrChunk = 64;
+constexpr int kRrLd = kRrChunk + 4;
+constexpr int kRrWaveFloats = 64 * kRrLd + kRrChunk;
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One lane's candidate distance from the staged rows: eight partial sums
+// over dims l, l + 8, ... (product, then add) continued chunk after chunk in
+// ascending dimension order, exactly as the unstaged loop.
 template <int R>
+__device__ __forceinline__ void staged_dists(const float* __restrict__ base, int64_t np,
+                                             int64_t stride, int d, const float* __restrict__ qv,
+                                             int n_cand, int lane, float* rows,
+                                             const int (&id)[R], float (&dist)[R]) {
+  float* qs = rows + 64 * kRrLd;
+  const int seg = lane & 15, sub = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    // source rows of this round: lane 16 t' + seg loads row 4 t + sub
+    int64_t src[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int row = 4 * t + sub;
+      const int x = __shfl(id[r], row, 64);
+      src[t] = (row + 64 * r < n_cand && x >= 0 && x < np) ? (int64_t)x * stride : -1;
+    }
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int j0 = 0; j0 < d; j0 += kRrChunk) {
+      float4 v[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+        v[t] = src[t] >= 0 ? *reinterpret_cast<const float4*>(base + src[t] + j0 + 4 * seg)
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 qq = *reinterpret_cast<const float4*>(qv + j0 + 4 * seg);
+      wave_lds_sync();  // the previous chunk's reads are done
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+        *reinterpret_cast<float4*>(rows + (4 * t + sub) * kRrLd + 4 * seg) = v[t];
+      if (sub == 0) *reinterpret_cast<float4*>(qs + 4 * seg) = qq;
+      wave_lds_sync();
+      const float* my = rows + lane * kRrLd;
+#pragma unroll
+      for (int h = 0; h < kRrChunk / 8; ++h) {
+        const float4 p0 = *reinterpret_cast<const float4*>(my + 8 * h);
+        const float4 p1 = *reinterpret_cast<const float4*>(my + 8 * h + 4);
+        const float4 q0 = *reinterpret_cast<const float4*>(qs + 8 * h);
+        const float4 q1 = *reinterpret_cast<const float4*>(qs + 8 * h + 4);
+        const float e[8] = {p0.x - q0.x, p0.y - q0.y, p0.z - q0.z, p0.w - q0.w,
+                            p1.x - q1.x, p1.y - q1.y, p1.z - q1.z, p1.w - q1.w};
+#pragma unroll
+        for (int l = 0; l < 8; ++l) acc[l] = acc[l] + e[l] * e[l];
+      }
+    }
+    const int c = lane + 64 * r;
+    if (c < n_cand && id[r] >= 0 && id[r] < np) {
+      float sum = 0.f;
+#pragma unroll
+      for (int l = 0; l < 8; ++l) sum = sum + acc[l];
+      dist[r] = sum;
+    }
+  }
+}
+
+template <int R, bool STAGED>
 __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ base, int64_t np,
                                                      int64_t stride, int d,
                                                      const float* __restrict__ Q, int64_t nq,
@@ -41,11 +111,19 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ b
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int c = lane + 64 * r;
-    id[r] = 0x7fffffff;
+    id[r] = c < n_cand ? cand[q * n_cand + c] : 0x7fffffff;
     dist[r] = INFINITY;
+  }
+  if constexpr (STAGED) {
+    extern __shared__ float rr_lds[];
+    staged_dists<R>(base, np, stride, d, qv, n_cand, lane,
+                    rr_lds + (threadIdx.x >> 6) * kRrWaveFloats, id, dist);
+  } else {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int c = lane + 64 * r;
     if (c < n_cand) {
-      const int32_t x = cand[q * n_cand + c];
-      id[r] = x;
+      const int32_t x = id[r];
       if (x >= 0 && x < np) {
         const float* pv = base + (int64_t)x * stride;
         float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -65,6 +143,7 @@ __global__ __launch_bounds__(256) void rerank_kernel(const float* __restrict__ b
         dist[r] = s;
       }
     }
+  }
   }
   for (int out = 0; out < k; ++out) {
     float bd = INFINITY;
@@ -106,11 +185,23 @@ hipError_t launch_rerank(const float* base, int64_t np, int64_t stride, int d, c
                          int64_t nq, int64_t q_stride, const int32_t* cand, int n_cand, int k,
                          int32_t* ids_out, float* dists_out, hipStream_t st) {
   if (nq <= 0) return hipSuccess;
-  const dim3 grid((unsigned)((nq + 3) / 4));
   const int r = (n_cand + 63) / 64;
-#define RR(N)                                                                                    \
-  hipLaunchKernelGGL(rerank_kernel<N>, grid, dim3(256), 0, st, base, np, stride, d, Q, nq,      \
-                     q_stride, cand, n_cand, k, ids_out, dists_out)
+  // rows in whole 64-dim chunks: coalesced 256-byte row segments staged
+  // through LDS (two waves per workgroup, 35 KB); otherwise a lane streams
+  // its own row
+  const bool staged = d % kRrChunk == 0;
+  const int wpb = staged ? 2 : 4;
+  const dim3 grid((unsigned)((nq + wpb - 1) / wpb));
+  const size_t lds = staged ? (size_t)wpb * kRrWaveFloats * sizeof(float) : 0;
+#define RR(N)                                                                              \
+  do {                                                                                     \
+    if (staged)                                                                            \
+      hipLaunchKernelGGL((rerank_kernel<N, true>), grid, dim3(64 * wpb), lds, st, base, np, \
+                         stride, d, Q, nq, q_stride, cand, n_cand, k, ids_out, dists_out);  \
+    else                                                                                   \
+      hipLaunchKernelGGL((rerank_kernel<N, false>), grid, dim3(64 * wpb), 0, st, base, np,  \
+                         stride, d, Q, nq, q_stride, cand, n_cand, k, ids_out, dists_out);  \
+  } while (0)
   if (r <= 1) RR(1);
   else if (r <= 2) RR(2);
   else if (r <= 4) RR(4);

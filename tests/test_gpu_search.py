@@ -367,3 +367,23 @@ def test_rerank_after_truncated_search(ctx, small_workload, dim, beam, k):
         print(f"[rerank] recall search-only {O.recall(cand[:, :10], w.gt):.4f} "
               f"reranked {O.recall(io, w.gt):.4f}", flush=True)
         assert O.recall(io, w.gt) >= O.recall(np.ascontiguousarray(cand[:, :10]), w.gt)
+
+
+@pytest.mark.parametrize("dim,beam", [(64, 64), (64, 128)])
+def test_rerank_960d(ctx, wide_workload, dim, beam):
+    """REORDER on 960-d rows (30 whole 128-byte lines per row): search on
+    the first 64 PCA columns, rerank on the original vectors, bit-exact
+    against the oracle."""
+    from rt_entry_amd import make_params
+    w = wide_workload
+    ctx.set_base(w.base, d=dim)
+    ctx.set_graph(w.graph_raw)
+    ctx.set_leaves(w.aabbs, w.pids)
+    ctx.set_rerank_base(w.raw_base)
+    q = np.ascontiguousarray(w.queries[:, :dim])
+    cand, _ = ctx.search(q, w.hits, make_params(k=beam, beam=beam, max_iter=40))
+    print(f"[rerank] 960-d dim={dim} beam={beam}", flush=True)
+    io, do = O.rerank(w.raw_base, w.raw_queries, cand, 10)
+    ih, dh = ctx.rerank(w.raw_queries, cand, 10)
+    assert_same(ih, io, "ids")
+    assert_same(dh.view(np.int32), do.view(np.int32), "dists")
