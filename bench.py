@@ -43,7 +43,6 @@ def log(*a):
 def prepare(workload: int, scale: float, dev, n_train: int = 0, keep_raw: bool = False):
     import torch
 
-    import oracle as O
     from rt_entry_amd import data as D
     from rt_entry_amd import gpuprep as G
     cfg = D.CONFIGS[workload]
@@ -190,7 +189,6 @@ def bench_uneven(args):
     checked equal; recall@k is reported per k."""
     import torch
 
-    import oracle as O
     from rt_entry_amd import Context, make_params
     from rt_entry_amd import data as D
     from rt_entry_amd import gpuprep as G
@@ -375,7 +373,6 @@ def main():
 
     import torch
 
-    import oracle as O
     from rt_entry_amd import Context, make_params
     from rt_entry_amd import data as D
     from rt_entry_amd import predictor as P
@@ -476,7 +473,7 @@ def main():
     def recall_of(p):
         step(p)
         torch.cuda.synchronize()
-        return O.recall(ids[:, :k].cpu().numpy(), W["gt"], 10)
+        return D.recall(ids[:, :k].cpu().numpy(), W["gt"], 10)
 
     beams = (args.beam,) if args.beam else (128, 256) if mixed else (32, 64, 128, 256)
     iters = (args.max_iter or 100,) if args.beam else (10, 15, 20, 25, 30, 35, 40, 45, 50, 55, 60,
@@ -546,7 +543,7 @@ def main():
             ctx.mlp_predict(PQv, fallback=kd_v, min_conf=mc, out=leaf_v)
             ids_v = ctx.search(PQv, leaf_v, pv, with_dists=False)
             torch.cuda.synchronize()
-            return O.recall(ids_v.cpu().numpy(), gt_v, 10)
+            return D.recall(ids_v.cpu().numpy(), gt_v, 10)
 
         t0 = time.time()
         # early stopping on the network's own leaves (no fallback): before the
@@ -646,7 +643,7 @@ def main():
                 rerank(p)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        rec = O.recall(ids[:, :k].cpu().numpy(), W["gt"], 10)
+        rec = D.recall(ids[:, :k].cpu().numpy(), W["gt"], 10)
         search_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
         pred_ms = None
         if use:
@@ -755,6 +752,7 @@ def main():
     # ---- CPU baseline: the oracle's search on the host cores, bounded sample
     cpu = None
     if not args.no_cpu_baseline:
+        import oracle as O  # the CPU baseline is the oracle's search, timed
         use_mlp[0] = head is ln
         ctx.project(W["Q"], out=pq)
         entries(use_mlp[0])

@@ -225,6 +225,19 @@ def knn_exact(X: np.ndarray, Q: np.ndarray, k: int, exclude_self: bool = False,
     return ids, dist
 
 
+def recall(ids: np.ndarray, gt: np.ndarray, at: int = 10, block: int = 16384) -> float:
+    """Recall@`at` as `graph_helper.cu:64-121` counts it: for each query, each
+    of the first min(gt_k, at) ground-truth ids found anywhere among the k
+    returned ids is a hit; the total is divided by nq * at."""
+    ids = np.asarray(ids)
+    gt = np.asarray(gt)[:, :at]
+    nq = ids.shape[0]
+    hits = 0
+    for s in range(0, nq, block):
+        hits += int((gt[s:s + block, :, None] == ids[s:s + block, None, :]).any(-1).sum())
+    return hits / nq / at
+
+
 def graph_from_knn(knn: np.ndarray, degree: int, rev_rank: int | None = None) -> np.ndarray:
     """Rank-pruned kNN graph with reverse edges (numpy).
 

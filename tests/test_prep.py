@@ -81,3 +81,23 @@ def test_k_beam_assignment():
     assert set(np.unique(k)) == {1, 10, 100}
     assert (b >= 32).all() and (b >= 128)[k == 100].all()
     assert ((b & (b - 1)) == 0).all()
+
+
+def test_recall_matches_oracle():
+    """bench.py's recall (package numpy) against the oracle's `orc_recall`
+    and closed forms: all found = 1, none found = 0, a ground truth shorter
+    than `at` still divides by `at`, hits counted once per ground-truth id."""
+    import oracle as O
+    from rt_entry_amd import data as D
+    rng = np.random.default_rng(7)
+    for nq, k, gk in ((1000, 10, 10), (300, 100, 100), (200, 1, 10), (150, 10, 5), (20000, 10, 10)):
+        ids = rng.integers(0, 40, (nq, k)).astype(np.int32)
+        gt = rng.integers(0, 40, (nq, gk)).astype(np.int32)
+        assert D.recall(ids, gt, 10) == O.recall(ids, gt, 10)
+    gt = np.arange(30, dtype=np.int32).reshape(3, 10)
+    assert D.recall(gt[:, ::-1], gt, 10) == 1.0
+    assert D.recall(gt + 100, gt, 10) == 0.0
+    assert D.recall(gt, gt[:, :5], 10) == 0.5
+    ids = np.zeros((1, 10), np.int32)
+    ids[0, 1] = 1
+    assert D.recall(ids, np.array([[1, 2, 3, 4, 5, 6, 7, 8, 9, 10]], np.int32), 10) == 0.1
