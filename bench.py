@@ -658,10 +658,10 @@ def main():
         search(entry, p, stats)
         torch.cuda.synchronize()
         st = stats.cpu().numpy().astype(np.int64)
-        D = 1 << (W["cfg"]["degree"] - 1).bit_length()
+        stride = 1 << (W["cfg"]["degree"] - 1).bit_length()
         # output bytes per query: (id, dist) for each of its results
         k_res = kq_h.astype(np.int64) if mixed else best[1] if sd else k
-        alg = float((4 * d_search * (st[:, 0] + 1) + 4 * D * st[:, 1] + 4 * best[1]
+        alg = float((4 * d_search * (st[:, 0] + 1) + 4 * stride * st[:, 1] + 4 * best[1]
                      + 8 * k_res).sum())
         return dict(qps=nq * args.steps / wall, ms=wall / args.steps * 1e3, recall=rec,
                     search_ms=search_ms, alg_bytes=alg, p=p, entry=entry, beam=best[1],
