@@ -18,6 +18,9 @@
  *    returns; device outputs are written in stream order on the context's
  *    stream (rte_synchronize() waits for them).
  *  - Ids are int32, distances are squared L2 in FP32.
+ *  - A batch of zero queries (nq == 0) returns RTE_OK and touches nothing;
+ *    its data pointers may be NULL (an empty device tensor has none).
+ *    Scalar arguments such as strides are still checked.
  */
 #ifndef RT_ENTRY_H
 #define RT_ENTRY_H

@@ -270,7 +270,7 @@ int rte_rerank(rte_ctx* c, const float* Q, int64_t nq, int64_t q_stride, const i
                int n_cand, int k, int32_t* ids_out, float* dists_out, int where) {
   if (!c) return RTE_EINVAL;
   if (!c->rr_base) return fail(c, RTE_ESTATE, "rte_set_rerank_base not called");
-  if (!Q || !cand || !ids_out || nq < 0 || q_stride < c->rr_d || q_stride % 4 != 0)
+  if ((nq > 0 && (!Q || !cand || !ids_out)) || nq < 0 || q_stride < c->rr_d || q_stride % 4 != 0)
     return fail(c, RTE_EINVAL, "bad rerank args (q_stride %lld, d %d)", (long long)q_stride,
                 c->rr_d);
   if (n_cand < 1 || n_cand > rte::kRerankMaxCand || k < 1 || k > n_cand)
@@ -444,7 +444,7 @@ int rte_set_pca(rte_ctx* c, const float* R, const float* mu, int d, int r_cols, 
 int rte_project(rte_ctx* c, const float* Q, int64_t nq, float* PQ, int where) {
   if (!c) return RTE_EINVAL;
   if (!c->pca_d) return fail(c, RTE_ESTATE, "rte_set_pca not called");
-  if (!Q || !PQ || nq < 0) return fail(c, RTE_EINVAL, "bad projection arguments");
+  if ((nq > 0 && (!Q || !PQ)) || nq < 0) return fail(c, RTE_EINVAL, "bad projection arguments");
   if (nq == 0) return RTE_OK;
   if (int r = set_device(c)) return r;
   const float* dq;
@@ -513,7 +513,8 @@ int rte_select_entry(rte_ctx* c, const float* PQ, int64_t nq, int stride, int32_
                      int where) {
   if (!c) return RTE_EINVAL;
   if (!c->n_nodes) return fail(c, RTE_ESTATE, "rte_set_leaves not called");
-  if (!PQ || !leaf_out || nq < 0 || stride < 3) return fail(c, RTE_EINVAL, "bad entry args");
+  if ((nq > 0 && (!PQ || !leaf_out)) || nq < 0 || stride < 3)
+    return fail(c, RTE_EINVAL, "bad entry args");
   if (nq == 0) return RTE_OK;
   if (int r = set_device(c)) return r;
   const float* dpq;
@@ -663,9 +664,10 @@ static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
     return fail(c, RTE_EINVAL, "graph has %lld rows, base %lld", (long long)c->graph_np,
                 (long long)c->np);
   if (p->algo == 1 && !c->pids) return fail(c, RTE_ESTATE, "rte_set_leaves not called");
-  if (!PQ || !ids_out || nq < 0 || q_stride < c->d)
+  if ((nq > 0 && (!PQ || !ids_out)) || nq < 0 || q_stride < c->d)
     return fail(c, RTE_EINVAL, "bad search args (q_stride %lld < d %d?)", (long long)q_stride, c->d);
-  if (p->algo == 1 && !leaf) return fail(c, RTE_EINVAL, "leaf array required for algo 1");
+  if (nq > 0 && p->algo == 1 && !leaf)
+    return fail(c, RTE_EINVAL, "leaf array required for algo 1");
   if (nq == 0) return RTE_OK;
   if (int r = set_device(c)) return r;
 

@@ -1178,7 +1178,8 @@ int rte_mlp_predict(rte_ctx* c, const float* X, int64_t n, const int32_t* fallba
                     float min_conf, int32_t* leaf_out, float* conf_out) {
   if (!c) return RTE_EINVAL;
   if (!c->mlp) return rte_fail(c, RTE_ESTATE, "rte_mlp_create not called");
-  if (!X || !leaf_out || n < 0) return rte_fail(c, RTE_EINVAL, "bad predict arguments");
+  if ((n > 0 && (!X || !leaf_out)) || n < 0)
+    return rte_fail(c, RTE_EINVAL, "bad predict arguments");
   if (n == 0) return RTE_OK;
   return predict_impl(c, X, c->mlp->dims[0], n, fallback, nullptr, 0, min_conf, leaf_out,
                       conf_out);
@@ -1189,7 +1190,7 @@ int rte_predict_entry(rte_ctx* c, const float* PQ, int64_t nq, int stride, float
   if (!c) return RTE_EINVAL;
   if (!c->mlp) return rte_fail(c, RTE_ESTATE, "rte_mlp_create not called");
   if (!c->n_nodes) return rte_fail(c, RTE_ESTATE, "rte_set_leaves not called");
-  if (!PQ || !leaf_out || nq < 0 || stride < std::max(3, c->mlp->dims[0]))
+  if ((nq > 0 && (!PQ || !leaf_out)) || nq < 0 || stride < std::max(3, c->mlp->dims[0]))
     return rte_fail(c, RTE_EINVAL, "bad predict_entry arguments (stride %d)", stride);
   if (nq == 0) return RTE_OK;
   return predict_impl(c, PQ, stride, nq, nullptr, PQ, stride, min_conf, leaf_out, conf_out);
