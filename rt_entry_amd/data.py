@@ -339,6 +339,59 @@ def read_bvecs(path, limit=100_000_000):
     return rows[:, 4:].astype(np.float32)
 
 
+def write_hvecs(path, x):
+    """int32 d, then the rows as float32 (file_read.cu:163-182)."""
+    x = np.ascontiguousarray(x, np.float32)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<i", x.shape[1]))
+        x.tofile(f)
+
+
+def read_hvecs(path):
+    """file_read.cu:163-182: n = (bytes / 4 - 1) / d rows after the header."""
+    raw = np.fromfile(path, np.int32)
+    d = int(raw[0])
+    n = (raw.size - 1) // d
+    return raw[1:1 + n * d].view(np.float32).reshape(n, d).copy()
+
+
+# the hh layout (file_read.cu:82-101) is the same header and rows
+write_hh, read_hh = write_hvecs, read_hvecs
+
+
+def write_txt(path, x):
+    """"n d" on the first line, then one comma-separated row per line
+    (file_read.cu:103-119 reads this)."""
+    x = np.asarray(x, np.float32)
+    with open(path, "w") as f:
+        f.write(f"{x.shape[0]} {x.shape[1]}\n")
+        for row in x:
+            f.write(",".join(repr(float(v)) for v in row) + "\n")
+
+
+def read_txt(path):
+    """file_read.cu:103-119: "n d", then n*d floats each followed by an
+    optional comma, whitespace anywhere between them."""
+    with open(path) as f:
+        head = f.readline().split()
+        n, d = int(head[0]), int(head[1])
+        toks = f.read().replace(",", " ").split()
+    if len(toks) < n * d:
+        raise ValueError(f"{path}: {len(toks)} values for {n}x{d}")
+    return np.array(toks[:n * d], dtype=np.float32).reshape(n, d)
+
+
+def read_data(path):
+    """Base or query vectors by file extension, in the order of
+    file_read.cu:16-80 (txt, fvecs, fbin, hvecs, bvecs, hh)."""
+    p = str(path)
+    for ext, fn in (("txt", read_txt), ("fvecs", read_fvecs), ("fbin", read_fbin),
+                    ("hvecs", read_hvecs), ("bvecs", read_bvecs), ("hh", read_hh)):
+        if p.endswith(ext):
+            return fn(p)
+    raise ValueError(f"unknown data file type: {p}")
+
+
 def write_fbin(path, x):
     x = np.ascontiguousarray(x, np.float32)
     with open(path, "wb") as f:

@@ -53,9 +53,39 @@ struct Dev {
 
 }  // namespace
 
-// file_read.cu:121-145 (fvecs), :147-161 (fbin), :184-211 (bvecs, at most
-// 100M rows), :82-101 (hh: int32 d then floats to EOF)
+// Text rows (file_read.cu:103-119): "n d", then n*d floats, each followed
+// by an optional comma; whitespace anywhere between them.
+void read_txt(const std::string& path, int& n, int& d, std::vector<float>& out) {
+  FILE* f = fopen(path.c_str(), "r");
+  if (!f) throw Error("cannot open " + path);
+  int nn = 0, dd = 0;
+  if (fscanf(f, "%d %d", &nn, &dd) != 2 || nn <= 0 || dd <= 0) {
+    fclose(f);
+    throw Error("bad txt header in " + path);
+  }
+  out.resize((size_t)nn * dd);
+  for (size_t i = 0; i < out.size(); ++i) {
+    if (fscanf(f, " %f", &out[i]) != 1) {
+      fclose(f);
+      throw Error("short txt data in " + path);
+    }
+    int c = fgetc(f);  // the separator, if it is a comma
+    if (c != ',' && c != EOF) ungetc(c, f);
+  }
+  fclose(f);
+  n = nn;
+  d = dd;
+}
+
+// By extension, in the reference's order (file_read.cu:16-80): txt, fvecs
+// (:121-145), fbin (:147-161), hvecs (:163-182: int32 d then floats,
+// n = (bytes/4 - 1)/d), bvecs (:184-211, at most 100M rows), hh (:82-101:
+// int32 d then floats to EOF).
 void read_data(const std::string& path, int& n, int& d, std::vector<float>& out) {
+  if (ends_with(path, "txt")) {
+    read_txt(path, n, d, out);
+    return;
+  }
   std::ifstream f = open_in(path);
   const int64_t size = file_size(f);
   if (ends_with(path, "fvecs")) {
@@ -76,6 +106,14 @@ void read_data(const std::string& path, int& n, int& d, std::vector<float>& out)
     n = hdr[0];
     d = hdr[1];
     if (n <= 0 || d <= 0 || 8 + 4LL * n * d > size) throw Error("bad fbin header in " + path);
+    out.resize((size_t)n * d);
+    f.read((char*)out.data(), 4LL * n * d);
+  } else if (ends_with(path, "hvecs")) {
+    int32_t dd = 0;
+    f.read((char*)&dd, 4);
+    if (dd <= 0) throw Error("bad hvecs header in " + path);
+    d = dd;
+    n = (int)((size / 4 - 1) / d);
     out.resize((size_t)n * d);
     f.read((char*)out.data(), 4LL * n * d);
   } else if (ends_with(path, "bvecs")) {
@@ -191,8 +229,13 @@ void Graph::Input() {  // graph.cu:73-112
   int qd = 0;
   read_data(query_path_, nq_, qd, queries_);
   if (qd != dim_) throw Error("query and base dimensions differ");
-  read_ivecs(gt_path_, nq_ == 0 ? nq_ : nq_, gt_k_, gt_);
-  if ((int64_t)gt_.size() / std::max(1, gt_k_) < nq_) throw Error("ground truth has too few rows");
+  int gt_n = 0;
+  read_ivecs(gt_path_, gt_n, gt_k_, gt_);
+  // graph_helper.cu:64-121 reads the first nq rows; a shorter file cannot
+  // score every query
+  if (gt_n < nq_)
+    throw Error("ground truth has " + std::to_string(gt_n) + " rows for " +
+                std::to_string(nq_) + " queries");
   if (search_dim_ <= 0 || search_dim_ > dim_) search_dim_ = dim_;
 }
 
@@ -300,7 +343,7 @@ void Graph::Search() {  // graph.cu:254-293
           "rte_set_rerank_base");
   Dev dq, dpq, dleaf, dids, ddist, dcand;
   hip_ok(hipMalloc(&dq.p, (size_t)nq_ * dim_ * 4), "hipMalloc queries");
-  hip_ok(hipMemcpy(dq.p, queries_.data(), (size_t)nq_ * dim_ * 4, hipMemcpyHostToDevice),
+  hip_ok(hipMemcpy(dq.p, queries_.data(), queries_.size() * 4, hipMemcpyHostToDevice),
          "copy queries");
   hip_ok(hipMalloc(&dpq.p, (size_t)nq_ * std::max(qdim, 3) * 4), "hipMalloc pq");
   hip_ok(hipMalloc(&dleaf.p, (size_t)nq_ * 4), "hipMalloc leaf");
@@ -328,13 +371,8 @@ void Graph::Search() {  // graph.cu:254-293
                    (int32_t*)(reorder_ ? dcand.p : dids.p), reorder_ ? nullptr : (float*)ddist.p,
                    nullptr, RTE_DEVICE),
         "rte_search");
-  check(rte_synchronize(ctx_), "sync");
-  const auto t1 = std::chrono::high_resolution_clock::now();
-  search_ms_ = std::chrono::duration<double>(t1 - t0).count() * 1000.0;
-  std::ostringstream os;
-  os << "time search: " << search_ms_ << " ms\n";  // Timing.h:41-46
-  log_line(os.str());
-  if (reorder_) {  // graph.cu:326-336, on the device
+  if (reorder_) {  // graph.cu:326-336, on the device; inside the search timer
+    check(rte_synchronize(ctx_), "sync");
     const auto t2 = std::chrono::high_resolution_clock::now();
     check(rte_rerank(ctx_, (const float*)dq.p, nq_, dim_, (const int32_t*)dcand.p, kk, topk_,
                      (int32_t*)dids.p, (float*)ddist.p, RTE_DEVICE),
@@ -342,10 +380,17 @@ void Graph::Search() {  // graph.cu:254-293
     check(rte_synchronize(ctx_), "sync");
     reorder_ms_ = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t2)
                       .count() * 1000.0;
-    std::ostringstream ro;
+    std::ostringstream ro;  // the nested timer stops first (Timing.h:41-46)
     ro << "time reorder: " << reorder_ms_ << " ms\n";
     log_line(ro.str());
   }
+  check(rte_synchronize(ctx_), "sync");
+  const auto t1 = std::chrono::high_resolution_clock::now();
+  // graph.cu:255-286: projection + entry + graph search (+ REORDER)
+  search_ms_ = std::chrono::duration<double>(t1 - t0).count() * 1000.0;
+  std::ostringstream os;
+  os << "time search: " << search_ms_ << " ms\n";  // Timing.h:41-46
+  log_line(os.str());
   results_.resize((size_t)nq_ * topk_);
   dists_.resize((size_t)nq_ * topk_);
   hits_.resize(nq_);

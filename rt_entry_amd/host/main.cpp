@@ -7,6 +7,9 @@
 //              [--topk 0] [--max_iter 100] [--search_width 1] [--ALGO 1]
 //              [--point_ratio 0.000128] [--expand_ratio 0.2] [--dim 0]
 //              [--mode 0] [--device 0] [--ids_out ids.ivecs] [--reorder 0]
+//   rte_search --check_file data.{txt,fvecs,fbin,hvecs,bvecs,hh}
+//              (reads one data file as the pipeline would and prints
+//               "n d sum", no GPU work)
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -14,6 +17,7 @@
 #include <fstream>
 #include <map>
 #include <string>
+#include <vector>
 
 #include "graph.h"
 
@@ -25,7 +29,8 @@ int main(int argc, char** argv) {
       {"ALGO", "1"},             {"point_ratio", "0.000128"}, {"expand_ratio", "0.2"},
       {"max_hits", "1"},         {"n_subspaces", "1"},     {"buffer_size", "200000"},
       {"dim", "0"},              {"mode", "0"},            {"device", "0"},
-      {"ids_out", ""},           {"out_file", "out.txt"},  {"reorder", "0"}};
+      {"ids_out", ""},           {"out_file", "out.txt"},  {"reorder", "0"},
+      {"check_file", ""}};
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     if (a.rfind("--", 0) != 0) {
@@ -48,6 +53,15 @@ int main(int argc, char** argv) {
     f[a] = v;
   }
   try {
+    if (!f["check_file"].empty()) {
+      int n = 0, d = 0;
+      std::vector<float> v;
+      rte_host::read_data(f["check_file"], n, d, v);
+      double sum = 0.0;
+      for (float x : v) sum += x;
+      printf("%d %d %.17g\n", n, d, sum);
+      return 0;
+    }
     {
       std::ofstream out(f["out_file"], std::ios::app);  // main.cpp:60-73
       out << "\n---------- n_candidates = " << f["n_candidates"] << "\tpoint_ratio = "

@@ -116,3 +116,54 @@ def test_driver_reorder_matches_oracle(tmp_path, int_workload, ctx):
     assert_same(ids, io, "driver reorder ids")
     text = open(tmp_path / "out.txt").read()
     assert "REORDER = 1" in text and "time reorder:" in text
+
+
+def test_driver_txt_and_hvecs_inputs(tmp_path, int_workload, ctx):
+    """Base and queries as .txt (file_read.cu:103-119) and .hvecs
+    (:163-182) give the same ids as the .fvecs run."""
+    from rt_entry_amd import data as D
+    w = int_workload
+    _write_inputs(tmp_path, w)
+    cache = tmp_path / "cache"
+    _run(tmp_path, cache)
+    ref = D.read_ivecs(tmp_path / "ids.ivecs")
+    for ext, writer in (("txt", D.write_txt), ("hvecs", D.write_hvecs)):
+        writer(tmp_path / f"base.{ext}", w.raw_base)
+        writer(tmp_path / f"query.{ext}", w.raw_queries)
+        cmd = [DRIVER, "--data_path", str(tmp_path / f"base.{ext}"), "--query_path",
+               str(tmp_path / f"query.{ext}"), "--gt_path", str(tmp_path / "gt.ivecs"),
+               "--graph_path", str(tmp_path / "graph.nsw"), "--data_name", str(cache),
+               "--n_candidates", "64", "--topk", "10", "--point_ratio", str(64 / 8000),
+               "--ids_out", str(tmp_path / f"ids_{ext}.ivecs"), "--out_file",
+               str(tmp_path / "out.txt")]
+        print("[driver]", " ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert_same(D.read_ivecs(tmp_path / f"ids_{ext}.ivecs"), ref, f"{ext} ids")
+
+
+def test_driver_ground_truth_rows(tmp_path, int_workload, ctx):
+    """A ground-truth file with more rows than queries scores the first nq
+    rows (graph_helper.cu:64-121); one with fewer rows is an error."""
+    from rt_entry_amd import data as D
+    w = int_workload
+    _write_inputs(tmp_path, w)
+    cache = tmp_path / "cache"
+    _run(tmp_path, cache)
+    ref = D.read_ivecs(tmp_path / "ids.ivecs")
+    nq = w.raw_queries.shape[0]
+    gt = w.gt[:, :100]
+    D.write_ivecs(tmp_path / "gt.ivecs", np.concatenate([gt, gt[:7]], axis=0))
+    _run(tmp_path, cache)
+    assert_same(D.read_ivecs(tmp_path / "ids.ivecs"), ref, "ids with a longer gt file")
+    text = open(tmp_path / "out.txt").read()
+    rec = float(text.rsplit("recall@10 = ", 1)[1].split()[0])
+    assert abs(rec - O.recall(ref, gt)) < 1e-6
+    D.write_ivecs(tmp_path / "gt.ivecs", gt[:nq - 3])
+    cmd = [DRIVER, "--data_path", str(tmp_path / "base.fvecs"), "--query_path",
+           str(tmp_path / "query.fvecs"), "--gt_path", str(tmp_path / "gt.ivecs"),
+           "--graph_path", str(tmp_path / "graph.nsw"), "--data_name", str(cache),
+           "--n_candidates", "64", "--topk", "10", "--point_ratio", str(64 / 8000),
+           "--out_file", str(tmp_path / "out.txt")]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "ground truth has" in r.stderr

@@ -73,6 +73,64 @@ def test_file_formats_roundtrip(tmp_path):
     raw[:, 4:] = rng.integers(0, 255, size=(3, 5))
     raw.tofile(tmp_path / "a.bvecs")
     assert_same(D.read_bvecs(tmp_path / "a.bvecs"), raw[:, 4:].astype(np.float32), "bvecs")
+    D.write_hvecs(tmp_path / "a.hvecs", x)
+    assert_same(D.read_hvecs(tmp_path / "a.hvecs"), x, "hvecs")
+    # hvecs: int32 d, then the rows (file_read.cu:163-182)
+    hv = np.fromfile(tmp_path / "a.hvecs", np.int32)
+    assert hv[0] == 5 and hv.size == 1 + 7 * 5
+    D.write_hh(tmp_path / "a.hh", x)
+    assert_same(D.read_hh(tmp_path / "a.hh"), x, "hh")
+    D.write_txt(tmp_path / "a.txt", x)
+    assert_same(D.read_txt(tmp_path / "a.txt"), x, "txt")  # repr round-trips float32
+    assert open(tmp_path / "a.txt").readline() == "7 5\n"
+    # separators as fscanf("%f,") accepts them: commas, or whitespace alone
+    (tmp_path / "b.txt").write_text("2 3\n1.5,2,-3e2\n4\n5.25 6\n")
+    assert_same(D.read_txt(tmp_path / "b.txt"),
+                np.array([[1.5, 2, -300], [4, 5.25, 6]], np.float32), "txt separators")
+    for ext in ("txt", "fvecs", "fbin", "hvecs", "hh"):
+        assert_same(D.read_data(tmp_path / f"a.{ext}"), x, f"read_data .{ext}")
+
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                      "rt_entry_amd", "rte_search")
+
+
+def test_driver_reads_every_data_format(tmp_path):
+    """The C++ driver's readers (host/graph.cpp read_data) on files written by
+    data.py, through `rte_search --check_file` (no GPU work): same n, d and
+    FP64 sum of the values for txt, fvecs, fbin, hvecs, bvecs and hh."""
+    import subprocess
+    from rt_entry_amd import data as D
+    if not os.path.exists(DRIVER):
+        import pytest
+        pytest.skip("driver not built")
+    rng = np.random.default_rng(3)
+    x = rng.integers(-50, 50, size=(33, 12)).astype(np.float32) / 4
+    D.write_txt(tmp_path / "a.txt", x)
+    D.write_fvecs(tmp_path / "a.fvecs", x)
+    D.write_fbin(tmp_path / "a.fbin", x)
+    D.write_hvecs(tmp_path / "a.hvecs", x)
+    D.write_hh(tmp_path / "a.hh", x)
+    xb = rng.integers(0, 255, size=(33, 12)).astype(np.uint8)
+    raw = np.zeros((33, 16), np.uint8)
+    raw[:, :4] = np.frombuffer(np.int32(12).tobytes(), np.uint8)
+    raw[:, 4:] = xb
+    raw.tofile(tmp_path / "a.bvecs")
+    (tmp_path / "b.txt").write_text("2 3\n1.5,2,-3e2\n4\n5.25 6\n")
+    cases = [(f"a.{e}", x) for e in ("txt", "fvecs", "fbin", "hvecs", "hh")]
+    cases += [("a.bvecs", xb.astype(np.float32)),
+              ("b.txt", np.array([[1.5, 2, -300], [4, 5.25, 6]], np.float32))]
+    for name, ref in cases:
+        r = subprocess.run([DRIVER, "--check_file", str(tmp_path / name)], capture_output=True,
+                           text=True, timeout=60)
+        assert r.returncode == 0, r.stderr
+        n, d, s = r.stdout.split()
+        # every value is a multiple of 1/4 below 2^10: the float sum is exact
+        assert (int(n), int(d)) == ref.shape and float(s) == float(ref.astype(np.float64).sum()), name
+    (tmp_path / "short.txt").write_text("3 2\n1,2\n3,4\n")
+    r = subprocess.run([DRIVER, "--check_file", str(tmp_path / "short.txt")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "short txt data" in r.stderr
 
 
 def test_k_beam_assignment():
