@@ -326,6 +326,35 @@ void orc_merge_asc(float* d, int32_t* id, int n, int mode) {
   free(a);
 }
 
+/* One batch into the beam: sort B descending (:888-919 / :1424-1453), then
+ * min-exchange the C = min(N, M) pairs L[N-C+i] / B[M-C+i] (:921-931 /
+ * :1455-1467; in the main loop a MAX batch element is skipped), then the
+ * half-cleaner merge of L (:933-958 / :1469-1493).  When C = N the pairs
+ * join L ascending with the N smallest of B descending, a bitonic sequence
+ * whose merge leaves L sorted with the N smallest of L and B. */
+static void merge_batch(pair_t* L, int N, pair_t* B, int M, int mode, int skip_max) {
+  const int C = N < M ? N : M; /* :80-83 n_compare */
+  sort_desc(B, M, mode);
+  for (int i = 0; i < C; ++i) {
+    int u = N - C + i;
+    if (skip_max && B[M - C + i].d == ORC_MAXD) continue;
+    if (key_gt(L[u], B[M - C + i], mode)) cswap(&L[u], &B[M - C + i]);
+  }
+  merge_asc(L, N, mode);
+}
+
+void orc_merge_batch(float* ld, int32_t* li, int N, float* bd, int32_t* bi, int M, int mode,
+                     int skip_max) {
+  pair_t* L = (pair_t*)malloc(sizeof(pair_t) * (size_t)(N + M));
+  pair_t* B = L + N;
+  for (int i = 0; i < N; ++i) L[i] = (pair_t){ld[i], li[i]};
+  for (int i = 0; i < M; ++i) B[i] = (pair_t){bd[i], bi[i]};
+  merge_batch(L, N, B, M, mode, skip_max);
+  for (int i = 0; i < N; ++i) { ld[i] = L[i].d; li[i] = L[i].id; }
+  for (int i = 0; i < M; ++i) { bd[i] = B[i].d; bi[i] = B[i].id; }
+  free(L);
+}
+
 /* ---- the search (graph_search.cuh:56-1964) --------------------------- */
 typedef struct {
   const float* base;
@@ -363,7 +392,6 @@ static void search_one(const space_t* S, const float* q, int64_t qidx, int hit, 
   const int D = 1 << S->shift;
   const int W = p->width;
   const int M = W * D;
-  const int C = N < M ? N : M;                    /* :80-83 n_compare */
   const int mode = p->mode;
   bloom_t bl = {bw, p->bloom_words, p->bloom_hashes, (uint32_t)p->bloom_words * 32u};
   int64_t E = 0, X = 0;
@@ -399,12 +427,7 @@ static void search_one(const space_t* S, const float* q, int64_t qidx, int hit, 
         B[i].d = orc_distance(S->base + (int64_t)B[i].id * S->stride, q, S->d);
         ++E;
       }
-    sort_desc(B, M, mode);
-    for (int i = 0; i < C; ++i) { /* :921-931, no MAX skip while seeding */
-      int u = N - C + i;
-      if (key_gt(L[u], B[M - C + i], mode)) cswap(&L[u], &B[M - C + i]);
-    }
-    merge_asc(L, N, mode);
+    merge_batch(L, N, B, M, mode, 0); /* no MAX skip while seeding */
   }
 
   /* main loop, :964-1532 */
@@ -456,13 +479,7 @@ static void search_one(const space_t* S, const float* q, int64_t qidx, int hit, 
         ++E;
       }
     }
-    sort_desc(B, M, mode);
-    for (int i = 0; i < C; ++i) { /* :1455-1467 */
-      int u = N - C + i;
-      if (B[M - C + i].d == ORC_MAXD) continue;
-      if (key_gt(L[u], B[M - C + i], mode)) cswap(&L[u], &B[M - C + i]);
-    }
-    merge_asc(L, N, mode);
+    merge_batch(L, N, B, M, mode, 1);
     for (chunk = 0; chunk < nchunks; ++chunk) { /* :1495-1516 */
       mask = scan_chunk(L, N, chunk, np, &czero);
       if (mask != 0) break;

@@ -105,6 +105,13 @@ int orc_search_uneven(const float* base, int64_t np, int64_t stride, int d,
  * network tests (graph_search.cuh:888-919 sort, :933-958 merge). */
 void orc_sort_desc(float* d, int32_t* id, int m, int mode);
 void orc_merge_asc(float* d, int32_t* id, int n, int mode);
+/* One batch into the beam, as the search does it: sort B (m entries)
+ * descending, min-exchange the min(n, m) pairs L[n-C+i] / B[m-C+i]
+ * (skip_max: skip a batch entry at MAX, the main loop's rule), merge L
+ * (n entries) ascending (graph_search.cuh:888-958, :1424-1493).  Both arrays
+ * are updated in place. */
+void orc_merge_batch(float* ld, int32_t* li, int n, float* bd, int32_t* bi, int m, int mode,
+                     int skip_max);
 
 /* Full-precision rerank (graph_helper.cu:146-213 Graph::parallel_reorder,
  * enabled by REORDER in graph.cu:326-336): distance of every candidate in

@@ -322,6 +322,68 @@ def test_search_complete_graph_is_exact_knn(mode):
     assert_same(ids, exact, "exact knn")
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("n,width,beam", [(60, 1, 32),   # degree 64 >= n-1: M = 64 > N
+                                          (33, 2, 32),   # degree 32, width 2: M = 64 > N
+                                          (40, 2, 32),   # degree 64, width 2: M = 128
+                                          (31, 1, 16),   # beams below 32 (P2 mask)
+                                          (31, 1, 8),
+                                          (20, 2, 4)])
+def test_search_complete_graph_wide_batch_is_exact_knn(mode, n, width, beam):
+    """Batch wider than the beam (C = N < M): the first expansion of the
+    single entry point puts every other point into B, and the min-exchange
+    must pair L[i] with the i-th of B's N smallest (B[M-N+i]) for the beam to
+    hold the exact N nearest.  A pairing with B[i] (B's largest, padding
+    included) keeps far points and fails here."""
+    rng = np.random.default_rng(16)
+    d = 24
+    X = rng.normal(size=(n, d)).astype(np.float32)
+    Q = rng.normal(size=(30, d)).astype(np.float32)
+    g = _complete_graph(n)
+    pids = np.array([[0]], np.int32)
+    hits = np.zeros(Q.shape[0], np.int32)
+    k = min(10, beam)
+    p = O.make_params(k=k, beam=beam, width=width, mode=mode)
+    ids, dists = O.search(X, g, Q, pids, hits, p)
+    d2 = ((Q[:, None, :].astype(np.float64) - X[None]) ** 2).sum(-1)
+    exact = np.argsort(d2, axis=1, kind="stable")[:, :k]
+    assert_same(ids, exact, "exact knn")
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_merge_batch_wide_batch_keeps_smallest_sorted(mode):
+    """Per-iteration invariant of the merge when C = N (batch at least as
+    wide as the beam): from any ascending beam and any batch, the merged
+    beam is sorted ascending and is exactly the N smallest of L and B (by
+    (dist, id); distinct distances so both comparators agree), and the
+    pairs leave B with the displaced entries.  Checked for every N, M power
+    of two with 1 <= N <= M <= 128, with and without MAX entries."""
+    rng = np.random.default_rng(17)
+    for lgn in range(0, 8):
+        for lgm in range(lgn, 8):
+            N, M = 1 << lgn, 1 << lgm
+            for trial in range(6):
+                vals = rng.permutation(4 * (N + M)).astype(np.float32)[:N + M] + 1
+                ids = rng.permutation(10 * (N + M))[:N + M].astype(np.int32)
+                ld, li = vals[:N].copy(), ids[:N].copy()
+                if trial % 2:
+                    ld[N // 2:] = O.MAXD  # part-filled beam
+                order = np.argsort(ld, kind="stable")
+                ld, li = ld[order], li[order]
+                bd, bi = vals[N:].copy(), ids[N:].copy()
+                if trial >= 3:
+                    bd[rng.random(M) < 0.3] = O.MAXD  # filtered candidates
+                skip = trial >= 4
+                Ld, Li, Bd, Bi = O.merge_batch(ld, li, bd, bi, mode, skip_max=skip)
+                assert (np.diff(Ld) >= 0).all(), (N, M, trial)
+                allp = sorted(zip(np.r_[ld, bd].tolist(), np.r_[li, bi].tolist()))
+                fin = [x for x in allp if x[0] < O.MAXD][:N]
+                got = sorted((a, b) for a, b in zip(Ld.tolist(), Li.tolist()) if a < O.MAXD)
+                assert got == fin, (N, M, trial)
+                # nothing lost: beam and batch together hold the same pairs
+                assert sorted(zip(np.r_[Ld, Bd].tolist(), np.r_[Li, Bi].tolist())) == allp
+
+
 def test_search_max_iter_zero_returns_sorted_entries():
     rng = np.random.default_rng(7)
     n, d, cap = 200, 32, 64
