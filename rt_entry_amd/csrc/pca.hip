@@ -168,11 +168,16 @@ __global__ __launch_bounds__(256) void pca_rotate_kernel(const float* __restrict
   }
 }
 
-// Symmetric eigen-decomposition (FP64) by Householder reduction to
-// tridiagonal form and the QL algorithm with implicit shifts (the EISPACK
-// tred2 / tql2 pair, as in Eigen's SelfAdjointEigenSolver that pca.cpp
-// uses), arranged so every inner loop walks rows: about 1.3 s at d = 960
-// and a few ms at d = 128.
+// Symmetric eigen-decomposition (FP64): the published EISPACK method,
+// tred2 (Householder reduction to tridiagonal form, accumulating the
+// transformations) followed by tql2 (QL iterations with implicit shifts on
+// the tridiagonal matrix), in the widely printed tred2/tqli formulation
+// (Martin, Reinsch and Wilkinson, Numer. Math. 11 and 12, 1968).  The
+// reference's pca.cpp uses Eigen's SelfAdjointEigenSolver, a different
+// formulation (tridiagonalisation then implicit symmetric QR steps with a
+// Wilkinson shift); both give the eigenpairs to FP64 rounding, which the
+// tests check against LAPACK.  Every inner loop walks rows: about 1.3 s at
+// d = 960 and a few ms at d = 128.
 //
 // Householder step: A (d x d, row-major, only its lower triangle used) is
 // destroyed; on return A holds Q with A_in = Q T Q^T, diag the diagonal of T
