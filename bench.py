@@ -36,6 +36,23 @@ WORKLOAD_NAMES = {1: "Gaussian mixture (64 centres)", 2: "integer-valued mixture
                   5: "integer-valued mixture (10k centres)"}
 
 
+SEARCH_SOURCES = ("search_impl.h", "search.h", "search.hip", "search_nr1.hip", "search_nr2.hip",
+                  "search_nr4.hip", "search_nr8.hip")
+
+
+def kernel_source_id() -> str:
+    """Hash of the search kernel's sources: counter figures recorded under
+    profiles/ apply only to the build they were measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in SEARCH_SOURCES:
+        p = os.path.join(ROOT, "rt_entry_amd", "csrc", f)
+        if os.path.exists(p):
+            h.update(f.encode())
+            h.update(open(p, "rb").read())
+    return h.hexdigest()[:16]
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -737,16 +754,25 @@ def main():
         del a, b
 
     # HBM bytes per launch of this setting, from rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE runs scaled by a copy of known size (null if not profiled)
-    traffic = None
+    # WRITE_SIZE runs scaled by a copy of known size; only a counter run of
+    # this exact kernel source, workload, setting and launch counts (null
+    # otherwise, with the reason)
+    traffic, traffic_note = None, "no counter run recorded"
     tf = os.path.join(ROOT, "profiles", "search_traffic.json")
+    key = dict(kernel_source=kernel_source_id(), workload=args.workload, scale=args.scale,
+               entry="learned" if head is ln else "kd-tree", beam=head["beam"],
+               max_iter=head["max_iter"], launch=head["launch"].split(" ")[0],
+               search_dims=d_search, mode=args.mode)
     if os.path.exists(tf):
-        entry = "learned" if head is ln else "kd-tree"
-        for tj in json.load(open(tf)):
-            if (tj.get("beam"), tj.get("max_iter"), tj.get("entry")) == (
-                    head["beam"], head["max_iter"], entry) and args.workload == 2 and \
-                    args.scale == 1.0 and not sd:
-                traffic = tj.get("hbm_bytes_per_launch")
+        recs = json.load(open(tf))
+        hit = [tj for tj in recs if all(tj.get(kk) == vv for kk, vv in key.items())]
+        if hit:
+            traffic = hit[0]["hbm_bytes_per_launch"]
+            traffic_note = f"rocprofv3 counters, {hit[0].get('summary')}"
+        elif any(tj.get("kernel_source") == key["kernel_source"] for tj in recs):
+            traffic_note = "no counter run of this setting with the current kernel source"
+        else:
+            traffic_note = "counter runs recorded are of another kernel source"
     achieved = head["alg_bytes"] / (head["search_ms"] * 1e-3)
 
     # ---- CPU baseline: the oracle's search on the host cores, bounded sample
@@ -755,7 +781,9 @@ def main():
         import oracle as O  # the CPU baseline is the oracle's search, timed
         use_mlp[0] = head is ln
         ctx.project(W["Q"], out=pq)
-        entries(use_mlp[0])
+        search(entries(use_mlp[0]), head["p"])  # the head's results, for the id check
+        if sd:
+            rerank(head["p"])
         torch.cuda.synchronize()
         pq_h = pq.cpu().numpy()
         ent_h = (learned if use_mlp[0] else hits).cpu().numpy()
@@ -764,14 +792,34 @@ def main():
         g_h = pad_graph(g_h, W["n"])
         po = O.make_params(k=head["beam"] if sd else k, beam=head["beam"],
                            max_iter=head["max_iter"], mode=args.mode)
-        threads = min(os.cpu_count() or 1, 16)
+        # every host thread this process is given: the GPU box grants one
+        # GPU's share of its cores through OMP_NUM_THREADS; nproc is recorded
+        nproc = os.cpu_count() or 1
+        threads = int(os.environ.get("OMP_NUM_THREADS") or nproc)
+        cpu_model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                cpu_model = next((row.split(":", 1)[1].strip() for row in f
+                                  if row.startswith("model name")), "")
+        except OSError:
+            pass
+        import tempfile
+        try:  # -O3 -march=native for this host, compiled here
+            native = O.build_native(tempfile.mkdtemp(prefix="orc_native_"))
+            build_note = "gcc -O3 -march=native -fopenmp -ffp-contract=off (built on this host)"
+        except Exception as e:  # noqa: BLE001
+            native = None
+            build_note = f"portable -O3 -march=x86-64-v3 build (native build failed: {e})"
         base_s = np.ascontiguousarray(W["base_h"][:, :sd]) if sd else W["base_h"]
         pq_s = np.ascontiguousarray(pq_h[:, :sd]) if sd else pq_h
+        cpu_ids = [None]
 
         def cpu_search(m):
-            ci, _ = O.search(base_s, g_h, pq_s[:m], W["pids"], ent_h[:m], po, n_threads=threads)
+            ci, _ = O.search(base_s, g_h, pq_s[:m], W["pids"], ent_h[:m], po, n_threads=threads,
+                             lib_path=native)
             if sd:
-                O.rerank(W["raw_h"], W["Q_h"][:m], ci, k, n_threads=threads)
+                ci, _ = O.rerank(W["raw_h"], W["Q_h"][:m], ci, k, n_threads=threads)
+            cpu_ids[0] = ci
 
         m = 256
         t = 0.0
@@ -788,11 +836,17 @@ def main():
             t0 = time.perf_counter()
             cpu_search(m)
             times.append(time.perf_counter() - t0)
+        # the baseline computes the same answers as the GPU step it is set beside
+        same = bool(np.array_equal(cpu_ids[0][:, :k], ids[:m, :k].cpu().numpy()))
         cpu = {"value": round(m / float(np.median(times)), 1), "unit": "queries/s",
                "cores": threads, "kind": "port",
                "sample": f"first {m} of the {nq} queries with the same PCA queries, entry "
                          f"leaves, graph and parameters as the headline run; search stage "
-                         f"only{' (+ rerank)' if sd else ''}; median of {reps} runs"}
+                         f"only{' (+ rerank)' if sd else ''}; median of {reps} runs",
+               "nproc": nproc, "cpu_model": cpu_model, "build": build_note,
+               "threads_source": ("OMP_NUM_THREADS (the host-thread share given to this "
+                                  "process)" if os.environ.get("OMP_NUM_THREADS") else "nproc"),
+               "ids_equal_gpu": same}
 
     def summary(r):
         if r is None:
@@ -845,6 +899,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+            "traffic_source": traffic_note, "traffic_key": key,
             "kernel": "search_kernel", "launch": head["launch"],
             "kernel_ms": round(head["search_ms"], 4),
             "algorithmic_bytes_per_launch": head["alg_bytes"],

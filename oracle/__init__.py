@@ -36,15 +36,28 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
-_lib = None
+def build_native(out_dir: str) -> str:
+    """The CPU baseline's build of the same sources: -O3 -march=native for the
+    host it runs on (compiled there, since a native build for another CPU can
+    use instructions this one lacks), contraction still off so its results
+    equal the portable oracle's.  Returns the library path."""
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "liboracle_native.so")
+    subprocess.run(["gcc", "-O3", "-march=native", "-fopenmp", "-ffp-contract=off", "-fPIC",
+                    "-std=c11", "-shared", "-o", out, os.path.join(_HERE, "oracle.c"), "-lm"],
+                   check=True)
+    return out
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+_libs = {}
+
+
+def lib(path: str | None = None):
+    path = path or _LIB_PATH
+    if path not in _libs:
+        if path == _LIB_PATH and not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
+        L = C.CDLL(path)
         P = C.c_void_p
         i64 = C.c_int64
         L.orc_hash_raw.argtypes = [C.c_int, C.c_uint32]
@@ -71,6 +84,7 @@ def lib():
         L.orc_search_uneven.restype = C.c_int
         L.orc_sort_desc.argtypes = [P, P, C.c_int, C.c_int]
         L.orc_merge_asc.argtypes = [P, P, C.c_int, C.c_int]
+        L.orc_merge_batch.argtypes = [P, P, C.c_int, P, P, C.c_int, C.c_int, C.c_int]
         L.orc_rerank_distance.argtypes = [P, P, C.c_int]
         L.orc_rerank_distance.restype = C.c_float
         L.orc_rerank.argtypes = [P, i64, i64, C.c_int, P, i64, i64, P, C.c_int, C.c_int, P, P,
@@ -78,8 +92,8 @@ def lib():
         L.orc_recall.argtypes = [P, i64, C.c_int, P, C.c_int, C.c_int]
         L.orc_recall.restype = C.c_double
         L.orc_num_threads.restype = C.c_int
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _p(a: np.ndarray):
@@ -176,6 +190,16 @@ def merge_asc(d, ids, mode=MODE_REF):
     return dd, ii
 
 
+def merge_batch(ld, li, bd, bi, mode=MODE_REF, skip_max=False):
+    """One batch (bd, bi) into the beam (ld, li) as the search does it:
+    returns the new beam and the batch after its sort and exchanges."""
+    L = [np.array(ld, np.float32, copy=True), np.array(li, np.int32, copy=True)]
+    B = [np.array(bd, np.float32, copy=True), np.array(bi, np.int32, copy=True)]
+    lib().orc_merge_batch(_p(L[0]), _p(L[1]), L[0].size, _p(B[0]), _p(B[1]), B[0].size, mode,
+                          int(skip_max))
+    return L[0], L[1], B[0], B[1]
+
+
 def default_bloom(mode: int, beam: int):
     """Filter size: the reference's fixed 512 words x 3 hashes
     (graph_search.cuh:4-5) or the auto_tune_bloom.h sizing for TUNED."""
@@ -193,10 +217,11 @@ def make_params(k=10, beam=64, width=1, max_iter=100, mode=MODE_REF, algo=1,
 
 
 def search(base, graph, queries, pids, hits, params: Params, n_threads: int = 0,
-           with_stats: bool = False):
+           with_stats: bool = False, lib_path: str | None = None):
     """Oracle beam search.  graph must already have a power-of-two row stride
     (padded with np).  Returns ids[nq,k], dists[nq,k] (and stats[nq,3] =
-    distance evaluations, graph rows read, iterations)."""
+    distance evaluations, graph rows read, iterations).  lib_path selects
+    another build of the same sources (the CPU baseline's native build)."""
     b = _c(base, np.float32)
     g = _c(graph, np.int32)
     q = _c(queries, np.float32)
@@ -210,7 +235,7 @@ def search(base, graph, queries, pids, hits, params: Params, n_threads: int = 0,
     ids = np.empty((nq, params.k), np.int32)
     dists = np.empty((nq, params.k), np.float32)
     st = np.empty((nq, 3), np.int64) if with_stats else None
-    rc = lib().orc_search(_p(b), b.shape[0], b.shape[1], q.shape[1] if q.shape[1] < b.shape[1]
+    rc = lib(lib_path).orc_search(_p(b), b.shape[0], b.shape[1], q.shape[1] if q.shape[1] < b.shape[1]
                           else b.shape[1], _p(g), shift, _p(q), nq, q.shape[1], _p(pid),
                           pid.shape[1], _p(h), C.byref(params), _p(ids), _p(dists),
                           _p(st) if st is not None else None, int(n_threads))
