@@ -53,6 +53,16 @@ def kernel_source_id() -> str:
     return h.hexdigest()[:16]
 
 
+GRID_CODES = {0: "resident", -1: "2x resident", -2: "balanced rounds", -3: "one per query"}
+
+
+def launch_desc(p) -> str:
+    """The batched launch shape of a tuned parameter set."""
+    grid = GRID_CODES.get(p.grid, str(p.grid))
+    return (f"batched wpq={max(1, p.wpq)} wg_waves={p.wg_waves} grid={grid} "
+            f"order={'leaf' if p.order == 1 else 'given'} bloom_words={p.bloom_words or 'default'}")
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -238,7 +248,7 @@ def bench_uneven(args):
     ctx.project(W["Q"], out=pq)
     ctx.select_entry(pq, out=hits)
     tuned, tms = ctx.autotune_batched(pq, hits, kd_t, bd_t, p0, k_stride=kmax)
-    log(f"autotune: wg_waves={tuned.wg_waves} grid={tuned.grid} {tms:.3f} ms")
+    log(f"autotune: {launch_desc(tuned)} {tms:.3f} ms")
     # per-class subsets (gathered inside the per-class step, as a caller
     # without the batched launch would have to)
     classes = sorted(set(zip(k_q.tolist(), b_q.tolist())))
@@ -318,7 +328,7 @@ def bench_uneven(args):
             "workload": (f"{W['n']}x{d} normalised 2000-centre mixture (workload 3), {nq} "
                          f"queries, degree-{W['cfg']['degree']} graph, {W['n_leaf']} kd-tree "
                          f"leaves, kd-tree entry, max_iter {max_iter}"),
-            "launch": f"batched wg_waves={tuned.wg_waves} grid={tuned.grid}",
+            "launch": launch_desc(tuned),
             "seeds": {what: D.seed(3, i) for i, what in
                       enumerate(("centres", "base", "queries", "train", "k_beam"))},
             "classes": len(cls), "search_kernel_ms": round(search_ms, 4),
@@ -455,7 +465,7 @@ def main():
     def with_k(p, kk):
         return make_params(k=kk, beam=p.beam, width=p.width, max_iter=p.max_iter, mode=p.mode,
                            algo=p.algo, bloom_words=p.bloom_words, bloom_hashes=p.bloom_hashes,
-                           wg_waves=p.wg_waves, grid=p.grid)
+                           wg_waves=p.wg_waves, grid=p.grid, wpq=p.wpq, order=p.order)
 
     def cand_bufs(beam):
         """Whole-beam outputs of the truncated search (rerank candidates)."""
@@ -629,7 +639,7 @@ def main():
                                                   k_stride=k_out)
             else:
                 tuned, tms = ctx.autotune_batched(pq, entry0, None, None, pt, k_stride=pt.k)
-            log(f"autotune: wg_waves={tuned.wg_waves} grid={tuned.grid} {tms:.3f} ms")
+            log(f"autotune: {launch_desc(tuned)} {tms:.3f} ms")
 
         def run_search(entry, stats=None):
             if batched and mixed:
@@ -683,7 +693,7 @@ def main():
         return dict(qps=nq * args.steps / wall, ms=wall / args.steps * 1e3, recall=rec,
                     search_ms=search_ms, alg_bytes=alg, p=p, entry=entry, beam=best[1],
                     max_iter=best[2], iters=float(st[:, 2].mean()), pred_ms=pred_ms,
-                    launch=(f"batched wg_waves={tuned.wg_waves} grid={tuned.grid}" if batched
+                    launch=(launch_desc(tuned) if batched
                             else "batched, default shape" if mixed else "grid"),
                     tuned=tuned, batched=batched, use=use)
 
@@ -761,7 +771,9 @@ def main():
     tf = os.path.join(ROOT, "profiles", "search_traffic.json")
     key = dict(kernel_source=kernel_source_id(), workload=args.workload, scale=args.scale,
                entry="learned" if head is ln else "kd-tree", beam=head["beam"],
-               max_iter=head["max_iter"], launch=head["launch"].split(" ")[0],
+               max_iter=head["max_iter"],
+               launch=("batched order=" + ("leaf" if head["tuned"].order == 1 else "given")
+                       if head["batched"] else "grid"),
                search_dims=d_search, mode=args.mode)
     if os.path.exists(tf):
         recs = json.load(open(tf))

@@ -50,17 +50,41 @@ typedef struct rte_ctx rte_ctx;
 /* Search parameters (graph.cu:32-64 Graph ctor, flags.cpp defaults). */
 typedef struct rte_params {
   int k;            /* results per query, 1 <= k <= beam (topk) */
-  int beam;         /* n_candidates; rounded up to a power of two, 32..512 */
+  int beam;         /* n_candidates; rounded up to a power of two, 1..512
+                       (the batched calls use at least 32) */
   int width;        /* search_width: graph rows expanded per iteration */
   int max_iter;     /* iteration cap (flags.cpp max_iter, default 100) */
   int mode;         /* RTE_MODE_REF or RTE_MODE_TUNED */
   int algo;         /* 1: leaf entry table; 0 or 2: per-lane LCG entries */
   int bloom_words;  /* 0: default (REF 512, TUNED auto_tune_bloom sizing) */
   int bloom_hashes; /* 0, or the mode's count: REF 3, TUNED 7 */
-  int wg_waves;     /* queries (wave64s) per workgroup, 1..4; 0: 4 */
-  int grid;         /* workgroups launched; 0: one per wg_waves queries
-                       (rte_search) or the resident maximum (batched) */
+  int wg_waves;     /* queries (wave64s) per workgroup, 1..4; 0: 4 (with
+                       wpq > 1 the workgroup is the wpq waves of one query) */
+  int grid;         /* workgroups launched, > 0; or RTE_GRID_* (0: one per
+                       query unit for rte_search, the resident maximum for
+                       the batched calls) */
+  int wpq;          /* wavefronts per query: 0 or 1 (one wave holds a query),
+                       2 or 4 (the waves split each batch's distances and
+                       meet at a workgroup barrier; for small batches) */
+  int order;        /* batched calls: RTE_ORDER_GIVEN (0) or
+                       RTE_ORDER_LEAF: queries sorted by entry leaf on the
+                       device, the sorted range cut into one piece per XCD
+                       (workgroups blockIdx % 8), so queries that gather
+                       the same rows share an L2; never changes results */
 } rte_params;
+
+/* rte_params.grid codes, resolved per call from the launched kernel's
+ * occupancy (resident workgroups per CU x CUs). */
+#define RTE_GRID_DEFAULT 0    /* rte_search: one workgroup per query unit;
+                                 batched: the resident maximum */
+#define RTE_GRID_RESIDENT2 (-1) /* twice the resident maximum */
+#define RTE_GRID_BALANCED (-2)  /* the fewest units that take the same number
+                                   of rounds through the queue as the resident
+                                   maximum */
+#define RTE_GRID_PER_QUERY (-3) /* one workgroup per query unit */
+
+#define RTE_ORDER_GIVEN 0
+#define RTE_ORDER_LEAF 1
 
 /* ---- context ---------------------------------------------------------- */
 /* Replaces Graph::Graph + cublasCreate (graph.cu:32-64). */
@@ -214,13 +238,18 @@ int rte_search_batched(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stri
                        int32_t* ids_out, float* dists_out, int where);
 
 /* Autotuner for rte_search_batched: times the batch (device buffers) under
- * every launch shape (wg_waves 1/2/4 x grid: one workgroup per wg_waves
- * queries, or the resident maximum x1/x2) and every
- * filter size in bloom_words[0..n_bloom) (0 = the mode's default), and
- * writes the fastest configuration to *best (a copy of *p with wg_waves,
- * grid and bloom_words set) and its time to *best_ms.  Launch shape never
- * changes results; the filter size can, so callers pass only sizes they
- * accept.  ids_out/dists_out receive the last run's results. */
+ * every launch shape -- wavefronts per query (wpq 1, 2, 4) x queries per
+ * workgroup (wg_waves 1, 2, 4 at wpq 1) x grid (RTE_GRID_PER_QUERY,
+ * RTE_GRID_DEFAULT, RTE_GRID_RESIDENT2, RTE_GRID_BALANCED, each resolved
+ * from the occupancy of the kernel launched) x query order (RTE_ORDER_GIVEN,
+ * RTE_ORDER_LEAF with leaf entries) -- and every filter size in
+ * bloom_words[0..n_bloom) (0 = the mode's default), and writes the fastest
+ * configuration to *best (a copy of *p with wpq, wg_waves, grid, order and
+ * bloom_words set) and its mean time to *best_ms.  A shape that fails is
+ * skipped; the call fails only when every shape did (with the first
+ * failure).  Launch shape and order never change results; the filter size
+ * can, so callers pass only sizes they accept.  ids_out/dists_out receive
+ * the chosen configuration's results. */
 int rte_autotune_batched(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stride,
                          const int32_t* leaf, const int32_t* k_per_q,
                          const int32_t* beam_per_q, const rte_params* p, int k_stride,

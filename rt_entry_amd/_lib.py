@@ -24,7 +24,8 @@ EXPORTS = [
     "rte_create", "rte_destroy", "rte_last_error", "rte_set_stream", "rte_get_stream",
     "rte_synchronize", "rte_set_base", "rte_set_graph", "rte_set_pca", "rte_project",
     "rte_set_leaves", "rte_build_kdtree", "rte_expand_aabbs", "rte_select_entry",
-    "rte_search", "rte_search_batched", "rte_autotune_batched", "rte_query", "rte_mlp_create", "rte_mlp_train_step",
+    "rte_search", "rte_search_batched", "rte_autotune_batched",
+    "rte_query", "rte_mlp_create", "rte_mlp_train_step",
     "rte_mlp_predict", "rte_predict_entry", "rte_set_rerank_base", "rte_rerank", "rte_knn", "rte_knn_rows", "rte_graph_from_knn", "rte_pca_fit", "rte_pca_rotate", "rte_build_kdtree_gpu", "rte_mlp_get", "rte_mlp_set", "rte_mlp_grad", "rte_mlp_set_input_norm", "rte_hash_parameter",
     "rte_search_supported", "rte_version",
 ]
@@ -39,7 +40,7 @@ class RteError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [(n, C.c_int) for n in (
         "k", "beam", "width", "max_iter", "mode", "algo", "bloom_words", "bloom_hashes",
-        "wg_waves", "grid")]
+        "wg_waves", "grid", "wpq", "order")]
 
 
 _lib = None

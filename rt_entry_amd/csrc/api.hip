@@ -110,7 +110,7 @@ int check_params(rte_ctx* c, const rte_params* p, bool per_query) {
   if (!per_query) {
     if (p->beam < 1 || p->beam > 512)
       return fail(c, RTE_EINVAL, "beam %d outside 1..512", p->beam);
-    if (p->k < 1 || p->k > pow2_ceil(std::max(p->beam, 32)))
+    if (p->k < 1 || p->k > pow2_ceil(p->beam))
       return fail(c, RTE_EINVAL, "k %d must be in 1..beam", p->k);
   }
   if (p->width < 1 || p->width > rte::kMaxWidth)
@@ -563,6 +563,7 @@ __global__ void batch_prep_kernel(BatchPrep b, int32_t* __restrict__ k_out,
     beam_out[q] = p2;
     cl = (31 - __clz(p2)) - 5;
   }
+  if (!counter) return;  // the keyed order counts for itself
   const int lane = threadIdx.x & 63;
   for (int c = 0; c < 5; ++c) {
     const uint64_t m = __ballot(cl == c);
@@ -570,6 +571,68 @@ __global__ void batch_prep_kernel(BatchPrep b, int32_t* __restrict__ k_out,
       atomicAdd(reinterpret_cast<unsigned long long*>(counter + 1 + c),
                 (unsigned long long)__popcll(m));
   }
+}
+
+// RTE_ORDER_LEAF: a counting sort of the batch by key = entry leaf, after
+// the beam class (longest first) in an uneven batch.  Leaves are numbered in
+// kd-tree DFS order, so neighbouring keys are neighbouring regions of the
+// PCA space, whose searches gather many of the same rows.
+struct KeySort {
+  const int32_t* leaf;    // entry leaves
+  const int32_t* beam;    // effective beams (uneven batch) or null
+  int64_t nq;
+  int n_leaf;
+  int n_keys;             // n_leaf, or 5 * n_leaf with beam classes
+  int32_t* hist;          // n_keys counts, zero on entry, left zero by the scan
+  int32_t* cursor;        // n_keys scatter cursors
+  int32_t* order;         // nq query ids by key
+  int64_t* seg;           // 16 * segments work counters, reset by the scan
+  int segments;
+};
+
+__device__ __forceinline__ int sort_key(const KeySort& k, int64_t q) {
+  int l = k.leaf ? k.leaf[q] : 0;
+  if (l < 0 || l >= k.n_leaf) l = 0;  // the search's own clamp of a bad leaf
+  if (!k.beam) return l;
+  const int cl = (31 - __clz(k.beam[q])) - 5;  // beams 32..512 -> 0..4
+  return (4 - cl) * k.n_leaf + l;
+}
+
+__global__ void key_hist_kernel(KeySort k) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < k.nq) atomicAdd(&k.hist[sort_key(k, q)], 1);
+}
+
+// one workgroup of 1024 threads: thread t scans keys [t*per, (t+1)*per)
+__global__ __launch_bounds__(1024) void key_scan_kernel(KeySort k) {
+  __shared__ int32_t part[2][1024];
+  const int t = threadIdx.x;
+  const int per = (k.n_keys + 1023) / 1024;
+  const int k0 = t * per, k1 = min(k.n_keys, k0 + per);
+  int32_t sum = 0;
+  for (int i = k0; i < k1; ++i) sum += k.hist[i];
+  part[0][t] = sum;
+  __syncthreads();
+  int src = 0;
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+    const int32_t v = part[src][t] + (t >= off ? part[src][t - off] : 0);
+    part[src ^ 1][t] = v;
+    __syncthreads();
+    src ^= 1;
+  }
+  int32_t run = part[src][t] - sum;  // exclusive prefix of this thread's range
+  for (int i = k0; i < k1; ++i) {
+    const int32_t h = k.hist[i];
+    k.cursor[i] = run;
+    k.hist[i] = 0;
+    run += h;
+  }
+  if (t < 16 * k.segments) k.seg[t] = 0;
+}
+
+__global__ void key_scatter_kernel(KeySort k) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < k.nq) k.order[atomicAdd(&k.cursor[sort_key(k, q)], 1)] = (int32_t)q;
 }
 
 // cursors: classes in descending beam order
@@ -607,16 +670,55 @@ __global__ void batch_order_kernel(const int32_t* __restrict__ beam, int64_t nq,
 
 // Batch preparation + persistent search, captured once per launch shape as a
 // hipGraph and replayed (counters are cleared by a kernel, not a memset).
+__global__ void zero_words_kernel(int32_t* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// The batch preparation kernels and the search, in stream order on c->stream
+// (captured by launch_batched_graph, or launched directly).
+static hipError_t enqueue_batched(rte_ctx* c, const rte::SearchArgs& a, const BatchPrep& bp,
+                                  const KeySort* ks, int beam_max, int grid, int wpb) {
+  const unsigned nb = (unsigned)((bp.nq + 255) / 256);
+  if (bp.k_in && !ks) {  // uneven batch, ordered by beam class
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, c->stream, a.work_counter);
+    hipLaunchKernelGGL(batch_prep_kernel, dim3(nb), dim3(256), 0, c->stream, bp,
+                       (int32_t*)a.k_per_q, (int32_t*)a.beam_per_q, a.work_counter);
+    hipLaunchKernelGGL(batch_offsets_kernel, dim3(1), dim3(1), 0, c->stream, a.work_counter);
+    hipLaunchKernelGGL(batch_order_kernel, dim3(nb), dim3(256), 0, c->stream, a.beam_per_q,
+                       bp.nq, a.work_counter, (int32_t*)a.order);
+  } else if (ks) {  // ordered by (beam class,) entry leaf
+    if (bp.k_in)
+      hipLaunchKernelGGL(batch_prep_kernel, dim3(nb), dim3(256), 0, c->stream, bp,
+                         (int32_t*)a.k_per_q, (int32_t*)a.beam_per_q, (int64_t*)nullptr);
+    hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
+    hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(1024), 0, c->stream, *ks);
+    hipLaunchKernelGGL(key_scatter_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
+  }
+  const hipError_t e1 = hipGetLastError();
+  if (e1 != hipSuccess) return e1;
+  return rte::launch_search(a, beam_max, grid, wpb, c->stream);
+}
+
 static int launch_batched_graph(rte_ctx* c, const rte::SearchArgs& a, const BatchPrep& bp,
-                                int beam_max, int grid, int wpb) {
-  std::vector<unsigned char> key(sizeof(a) + sizeof(bp) + 4 * sizeof(int) + sizeof(void*));
+                                const KeySort* ks, int beam_max, int grid, int wpb) {
+  KeySort ksv{};
+  if (ks) ksv = *ks;
+  std::vector<unsigned char> key(sizeof(a) + sizeof(bp) + sizeof(ksv) + 5 * sizeof(int) +
+                                 sizeof(void*));
   unsigned char* k = key.data();
   memcpy(k, &a, sizeof(a));
-  memcpy(k + sizeof(a), &bp, sizeof(bp));
-  const int extra[4] = {beam_max, grid, wpb, c->device};
-  memcpy(k + sizeof(a) + sizeof(bp), extra, sizeof(extra));
+  k += sizeof(a);
+  memcpy(k, &bp, sizeof(bp));
+  k += sizeof(bp);
+  memcpy(k, &ksv, sizeof(ksv));
+  k += sizeof(ksv);
+  const int extra[5] = {beam_max, grid, wpb, c->device, ks != nullptr};
+  memcpy(k, extra, sizeof(extra));
+  k += sizeof(extra);
   void* sp = (void*)c->stream;
-  memcpy(k + sizeof(a) + sizeof(bp) + sizeof(extra), &sp, sizeof(sp));
+  memcpy(k, &sp, sizeof(sp));
   if (!c->gexec || key != c->gkey) {
     if (c->gexec) {
       (void)hipGraphExecDestroy(c->gexec);
@@ -624,21 +726,11 @@ static int launch_batched_graph(rte_ctx* c, const rte::SearchArgs& a, const Batc
     }
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed), "begin capture");
-    const unsigned nb = (unsigned)((bp.nq + 255) / 256);
-    if (bp.k_in) {  // uneven batch (a uniform one needs no preparation)
-      hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, c->stream, a.work_counter);
-      hipLaunchKernelGGL(batch_prep_kernel, dim3(nb), dim3(256), 0, c->stream, bp,
-                         (int32_t*)a.k_per_q, (int32_t*)a.beam_per_q, a.work_counter);
-      hipLaunchKernelGGL(batch_offsets_kernel, dim3(1), dim3(1), 0, c->stream, a.work_counter);
-      hipLaunchKernelGGL(batch_order_kernel, dim3(nb), dim3(256), 0, c->stream, a.beam_per_q,
-                         bp.nq, a.work_counter, (int32_t*)a.order);
-    }
-    const hipError_t e1 = hipGetLastError();
-    const hipError_t e2 = rte::launch_search(a, beam_max, grid, wpb, c->stream);
+    const hipError_t e2 = enqueue_batched(c, a, bp, ks, beam_max, grid, wpb);
     const hipError_t e3 = hipStreamEndCapture(c->stream, &g);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+    if (e2 != hipSuccess || e3 != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
-      return hip_fail(c, e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3, "capture batched");
+      return hip_fail(c, e2 != hipSuccess ? e2 : e3, "capture batched");
     }
     const hipError_t e4 = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -652,10 +744,83 @@ static int launch_batched_graph(rte_ctx* c, const rte::SearchArgs& a, const Batc
   return RTE_OK;
 }
 
+// Query ranges of RTE_ORDER_LEAF: one per XCD (workgroups are dealt to the
+// eight XCDs of the part round-robin by blockIdx)
+constexpr int kSegments = 8;
+
+// Workgroup count for a grid code (rte_params.grid), from the occupancy of
+// the kernel variant actually launched.
+static int resolve_grid(rte_ctx* c, const rte::SearchArgs& a, int beam_max, int wpb, int code,
+                        int64_t nq, bool batched, int* grid) {
+  const int qpb = a.wpq > 1 ? 1 : wpb;  // queries per workgroup
+  const int64_t per_query = std::min<int64_t>((nq + qpb - 1) / qpb, 1 << 20);
+  if (code > 0) {
+    *grid = code;
+    return RTE_OK;
+  }
+  if (code == RTE_GRID_PER_QUERY || (code == RTE_GRID_DEFAULT && !batched)) {
+    *grid = (int)per_query;
+    return RTE_OK;
+  }
+  int occ = 0, cus = 0;
+  HIPCHK(c, rte::search_occupancy(a, beam_max, wpb, &occ), "occupancy");
+  HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device),
+         "CU count");
+  const int64_t resident = (int64_t)std::max(1, occ) * std::max(1, cus);
+  int64_t g = resident;
+  if (code == RTE_GRID_RESIDENT2) {
+    g = 2 * resident;
+  } else if (code == RTE_GRID_BALANCED) {
+    // every round through the queue full: 10k queries on 4096 resident
+    // units run 3334 units for 3 rounds instead of 4096 + 4096 + 1808
+    const int64_t units = resident * qpb;
+    const int64_t rounds = (nq + units - 1) / units;
+    const int64_t need = (nq + rounds - 1) / rounds;
+    g = (need + qpb - 1) / qpb;
+  }
+  *grid = (int)std::max<int64_t>(1, std::min<int64_t>(g, 1 << 20));
+  return RTE_OK;
+}
+
+// Device buffers of a batched launch.
+struct BatchBufs {
+  int64_t* counter;  // 16: work counter (+ class counts and cursors)
+  int64_t* seg;      // 16 * kSegments range counters
+  int32_t* hist;     // n_keys
+  int32_t* cursor;   // n_keys
+  int32_t* k;        // nq effective k
+  int32_t* beam;     // nq effective beams
+  int32_t* order;    // nq query order
+};
+
+// Batched-launch buffers carved from one caller-provided device array
+// (layout in ctx.h, BatchScratch), for calls that must not allocate.
+static int scratch_bufs(rte_ctx* c, const rte::BatchScratch& sc, int64_t nq, int n_keys,
+                        int where, BatchBufs* bb) {
+  if (where != RTE_DEVICE)
+    return fail(c, RTE_EINVAL, "a call without allocation takes device buffers only");
+  if (nq > sc.max_queries)
+    return fail(c, RTE_EINVAL, "%lld queries above the limit %lld of this call",
+                (long long)nq, (long long)sc.max_queries);
+  if (n_keys > sc.max_keys)
+    return fail(c, RTE_EINVAL, "%d leaf keys above the limit %d of this call", n_keys,
+                sc.max_keys);
+  int32_t* base = sc.base;
+  bb->counter = reinterpret_cast<int64_t*>(base);
+  bb->seg = reinterpret_cast<int64_t*>(base + 2 * 16);
+  bb->hist = base + rte::kScratchCounterWords;
+  bb->cursor = bb->hist + sc.max_keys;
+  bb->k = bb->cursor + sc.max_keys;
+  bb->beam = bb->k + sc.max_queries;
+  bb->order = bb->beam + sc.max_queries;
+  return RTE_OK;
+}
+
 static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride,
                          const int32_t* leaf, const int32_t* k_per_q, const int32_t* beam_per_q,
                          const rte_params* p, int k_stride, int32_t* ids_out, float* dists_out,
-                         int32_t* stats_out, int where, bool batched) {
+                         int32_t* stats_out, int where, bool batched,
+                         const rte::BatchScratch* na_scratch = nullptr) {
   const bool uneven = beam_per_q != nullptr;
   if (int r = check_params(c, p, uneven)) return r;
   if (!c->base) return fail(c, RTE_ESTATE, "rte_set_base not called");
@@ -691,8 +856,9 @@ static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
   int beam_max;
   std::vector<int32_t> hk, hb;
   if (!uneven) {
+    // graph.cu:53: any n_candidates, rounded up to a power of two (beams
+    // below 32 use the first min(beam, 32) bits of the expansion mask, P2)
     const int beam = pow2_ceil(std::max(p->beam, 1));
-    if (beam < 32) return fail(c, RTE_EUNSUP, "beam below 32 is not compiled");
     a.beam = beam;
     a.k = p->k;
     a.k_stride = p->k;
@@ -751,64 +917,110 @@ static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
     if (int r = stage_out(c, c->s_aux2, dists_out, (size_t)nq * a.k_stride, where, &dd)) return r;
   if (stats_out)
     if (int r = stage_out(c, c->s_aux3, stats_out, (size_t)nq * 3, where, &dst)) return r;
-  const int wpb = p->wg_waves > 0 ? p->wg_waves : 4;
+  if (p->wpq != 0 && p->wpq != 1 && p->wpq != 2 && p->wpq != 4)
+    return fail(c, RTE_EINVAL, "wpq %d must be 0, 1, 2 or 4", p->wpq);
+  a.wpq = std::max(1, p->wpq);
+  int wpb = p->wg_waves > 0 ? p->wg_waves : 4;
   if (wpb > 4) return fail(c, RTE_EINVAL, "wg_waves %d outside 1..4", p->wg_waves);
-  if (p->grid < 0) return fail(c, RTE_EINVAL, "grid < 0");
+  if (a.wpq > 1) {  // the workgroup is the waves of one query
+    if (p->wg_waves > 0 && p->wg_waves != a.wpq)
+      return fail(c, RTE_EINVAL, "wg_waves %d must be 0 or wpq %d", p->wg_waves, a.wpq);
+    wpb = a.wpq;
+  }
+  if (p->grid < RTE_GRID_PER_QUERY) return fail(c, RTE_EINVAL, "unknown grid code %d", p->grid);
+  if (p->order != RTE_ORDER_GIVEN && p->order != RTE_ORDER_LEAF)
+    return fail(c, RTE_EINVAL, "unknown order %d", p->order);
   a.queries = dq;
   a.hits = dleaf;
   a.ids_out = dids;
   a.dists_out = dd;
   a.stats_out = dst;
+  // the leaf order needs leaves; a batch of random entries keeps its order
+  const bool by_leaf = batched && p->order == RTE_ORDER_LEAF && p->algo == 1 && c->n_leaf > 0;
   if (!batched) {
-    const int64_t blocks = (nq + wpb - 1) / wpb;
-    const int grid = p->grid > 0 ? p->grid : (int)std::min<int64_t>(blocks, 1 << 20);
+    int grid = 0;
+    if (int r = resolve_grid(c, a, beam_max, wpb, p->grid, nq, false, &grid)) return r;
     HIPCHK(c, rte::launch_search(a, beam_max, grid, wpb, c->stream), "search launch");
-  } else if (!uneven) {
-    // uniform batch: the captured graph is the search kernel alone; waves
-    // take queries from the self-clearing work counter (zeroed once here)
-    if (!c->b_counter.p) {
-      HIPCHK(c, c->b_counter.reserve(16 * sizeof(int64_t)), "hipMalloc(counter)");
-      HIPCHK(c, hipMemsetAsync(c->b_counter.p, 0, 16 * sizeof(int64_t), c->stream), "counter");
-    }
-    a.work_counter = c->b_counter.as<int64_t>();
-    int grid = p->grid;
-    if (grid == 0) {
-      int occ = 0, cus = 0;
-      HIPCHK(c, rte::search_occupancy(a, beam_max, wpb, &occ), "occupancy");
-      HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device),
-             "CU count");
-      grid = std::max(1, occ) * std::max(1, cus);
-    }
-    BatchPrep bp{nullptr, nullptr, nq, k_stride, beam_max};
-    if (int r = launch_batched_graph(c, a, bp, beam_max, grid, wpb)) return r;
   } else {
-    // persistent launch: waves pull queries from a counter, longest beams
-    // first (order built on the device), so mixed beams balance across CUs
-    HIPCHK(c, c->b_k.reserve(nq * sizeof(int32_t)), "hipMalloc(k_per_q)");
-    HIPCHK(c, c->b_beam.reserve(nq * sizeof(int32_t)), "hipMalloc(beam_per_q)");
-    HIPCHK(c, c->b_order.reserve(nq * sizeof(int32_t)), "hipMalloc(order)");
-    if (!c->b_counter.p) {
-      HIPCHK(c, c->b_counter.reserve(16 * sizeof(int64_t)), "hipMalloc(counter)");
-      HIPCHK(c, hipMemsetAsync(c->b_counter.p, 0, 16 * sizeof(int64_t), c->stream), "counter");
+    const int n_keys = by_leaf ? c->n_leaf * (uneven ? 5 : 1) : 0;
+    BatchBufs bb{};
+    if (na_scratch) {  // the caller's no-allocation call: the static scratch
+      if (int r = scratch_bufs(c, *na_scratch, nq, n_keys, where, &bb)) return r;
+    } else {
+      if (!c->b_counter.p) {
+        HIPCHK(c, c->b_counter.reserve(16 * sizeof(int64_t)), "hipMalloc(counter)");
+        HIPCHK(c, hipMemsetAsync(c->b_counter.p, 0, 16 * sizeof(int64_t), c->stream), "counter");
+      }
+      bb.counter = c->b_counter.as<int64_t>();
+      if (uneven || by_leaf) {
+        HIPCHK(c, c->b_order.reserve(nq * sizeof(int32_t)), "hipMalloc(order)");
+        bb.order = c->b_order.as<int32_t>();
+      }
+      if (uneven) {
+        HIPCHK(c, c->b_k.reserve(nq * sizeof(int32_t)), "hipMalloc(k_per_q)");
+        HIPCHK(c, c->b_beam.reserve(nq * sizeof(int32_t)), "hipMalloc(beam_per_q)");
+        bb.k = c->b_k.as<int32_t>();
+        bb.beam = c->b_beam.as<int32_t>();
+      }
+      if (by_leaf) {
+        HIPCHK(c, c->b_cursor.reserve((size_t)n_keys * sizeof(int32_t)), "hipMalloc(cursor)");
+        if (c->b_hist.bytes < (size_t)n_keys * sizeof(int32_t)) {
+          HIPCHK(c, c->b_hist.reserve((size_t)n_keys * sizeof(int32_t)), "hipMalloc(hist)");
+          HIPCHK(c, hipMemsetAsync(c->b_hist.p, 0, c->b_hist.bytes, c->stream), "hist");
+        }
+        HIPCHK(c, c->b_seg.reserve(16 * kSegments * sizeof(int64_t)), "hipMalloc(ranges)");
+        bb.hist = c->b_hist.as<int32_t>();
+        bb.cursor = c->b_cursor.as<int32_t>();
+        bb.seg = c->b_seg.as<int64_t>();
+      }
     }
-    const int32_t* kin;
-    const int32_t* bin;
-    if (int r = stage_in(c, c->b_kin, k_per_q, (size_t)nq, where, &kin)) return r;
-    if (int r = stage_in(c, c->b_bin, beam_per_q, (size_t)nq, where, &bin)) return r;
-    a.k_per_q = c->b_k.as<int32_t>();
-    a.beam_per_q = c->b_beam.as<int32_t>();
-    a.order = c->b_order.as<int32_t>();
-    a.work_counter = c->b_counter.as<int64_t>();
-    BatchPrep bp{kin, bin, nq, k_stride, beam_max};
-    int grid = p->grid;
-    if (grid == 0) {
-      int occ = 0, cus = 0;
-      HIPCHK(c, rte::search_occupancy(a, beam_max, wpb, &occ), "occupancy");
-      HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device),
-             "CU count");
-      grid = std::max(1, occ) * std::max(1, cus);
+    a.work_counter = bb.counter;
+    BatchPrep bp{nullptr, nullptr, nq, k_stride, beam_max};
+    if (uneven) {
+      // persistent launch: waves pull queries from a counter, longest beams
+      // first (order built on the device), so mixed beams balance across CUs
+      const int32_t* kin;
+      const int32_t* bin;
+      if (int r = stage_in(c, c->b_kin, k_per_q, (size_t)nq, where, &kin)) return r;
+      if (int r = stage_in(c, c->b_bin, beam_per_q, (size_t)nq, where, &bin)) return r;
+      a.k_per_q = bb.k;
+      a.beam_per_q = bb.beam;
+      a.order = bb.order;
+      bp.k_in = kin;
+      bp.beam_in = bin;
     }
-    if (int r = launch_batched_graph(c, a, bp, beam_max, grid, wpb)) return r;
+    // a uniform batch in the given order is the search kernel alone; its
+    // waves take queries from the self-clearing work counter
+    KeySort ks{};
+    if (by_leaf) {
+      a.order = bb.order;
+      a.work_counter = bb.seg;
+      a.segments = kSegments;
+      ks.leaf = dleaf;
+      ks.beam = uneven ? a.beam_per_q : nullptr;
+      ks.nq = nq;
+      ks.n_leaf = c->n_leaf;
+      ks.n_keys = n_keys;
+      ks.hist = bb.hist;
+      ks.cursor = bb.cursor;
+      ks.order = bb.order;
+      ks.seg = bb.seg;
+      ks.segments = kSegments;
+    }
+    int grid = 0;
+    if (int r = resolve_grid(c, a, beam_max, wpb, p->grid, nq, true, &grid)) return r;
+    if (na_scratch) {
+      // direct launches; what must start at zero is cleared by a kernel:
+      // the counter block, the range counters and the key histogram
+      const int64_t nz = rte::kScratchCounterWords + n_keys;
+      hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0,
+                         c->stream, na_scratch->base, nz);
+      HIPCHK(c, enqueue_batched(c, a, bp, by_leaf ? &ks : nullptr, beam_max, grid, wpb),
+             "batched launch");
+    } else if (int r = launch_batched_graph(c, a, bp, by_leaf ? &ks : nullptr, beam_max, grid,
+                                            wpb)) {
+      return r;
+    }
   }
   if (int r = finish_out(c, ids_out, dids, (size_t)nq * a.k_stride, where)) return r;
   if (dists_out)
@@ -837,6 +1049,21 @@ int rte_search_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride
                        dists_out, nullptr, where, true);
 }
 
+}  // extern "C"
+
+int rte_search_batched_in(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride,
+                          const int32_t* leaf, const int32_t* k_per_q, const int32_t* beam_per_q,
+                          const rte_params* p, int k_stride, int32_t* ids_out, float* dists_out,
+                          const rte::BatchScratch& sc) {
+  if (!c) return RTE_EINVAL;
+  if (!k_per_q != !beam_per_q)
+    return fail(c, RTE_EINVAL, "k_per_q and beam_per_q: both or neither");
+  return search_common(c, PQ, nq, q_stride, leaf, k_per_q, beam_per_q, p, k_stride, ids_out,
+                       dists_out, nullptr, RTE_DEVICE, true, &sc);
+}
+
+extern "C" {
+
 int rte_autotune_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride,
                          const int32_t* leaf, const int32_t* k_per_q,
                          const int32_t* beam_per_q, const rte_params* p, int k_stride,
@@ -846,73 +1073,74 @@ int rte_autotune_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
   if (!p || !best || !best_ms || !bloom_words || n_bloom < 1)
     return fail(c, RTE_EINVAL, "bad autotune arguments");
   if (int r = set_device(c)) return r;
-  int cus = 0;
-  HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device),
-         "CU count");
+  // candidates: waves per query x queries per workgroup x grid code x query
+  // order x filter size; every grid code is resolved by the call itself from
+  // the occupancy of the kernel it launches
+  std::vector<rte_params> cands;
+  const int orders[2] = {RTE_ORDER_GIVEN, RTE_ORDER_LEAF};
+  for (int bi = 0; bi < n_bloom; ++bi)
+    for (int wpq : {1, 2, 4})
+      for (int wpb : {1, 2, 4}) {
+        if (wpq > 1 && wpb != wpq) continue;
+        for (int g : {RTE_GRID_PER_QUERY, RTE_GRID_DEFAULT, RTE_GRID_RESIDENT2,
+                      RTE_GRID_BALANCED})
+          for (int o = 0; o < (p->algo == 1 ? 2 : 1); ++o) {
+            rte_params t = *p;
+            t.bloom_words = bloom_words[bi];
+            t.wpq = wpq;
+            t.wg_waves = wpb;
+            t.grid = g;
+            t.order = orders[o];
+            cands.push_back(t);
+          }
+      }
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0), "event");
   HIPCHK(c, hipEventCreate(&e1), "event");
   float best_t = 1e30f;
-  rte_params bp = *p;
-  int rc = RTE_OK;
-  for (int bi = 0; bi < n_bloom && rc == RTE_OK; ++bi)
-    for (int wpb : {1, 2, 4})
-      // 0: one workgroup per wpb queries; 1, 2: the resident grid times
-      // mult; -1: the fewest waves that take the same number of rounds
-      // through the queue as the resident grid (every round full: 10k
-      // queries on 4096 resident waves run 3334 waves, 3 full rounds,
-      // instead of 4096 + 4096 + 1808)
-      for (int mult : {0, 1, 2, -1}) {
-        rte_params t = *p;
-        t.bloom_words = bloom_words[bi];
-        t.wg_waves = wpb;
-        t.grid = 0;
-        // resident grid for this shape, times mult
-        if ((rc = rte_search_batched(c, PQ, nq, q_stride, leaf, k_per_q, beam_per_q, &t, k_stride,
-                                     ids_out, dists_out, RTE_DEVICE)) != RTE_OK)
-          break;
-        rte::SearchArgs probe{};
-        probe.d = c->d;
-        probe.shift = c->shift;
-        probe.width = t.width;
-        probe.mode = t.mode;
-        int bw = 0, bh = 0;
-        bloom_for(&t, 512, &bw, &bh);
-        probe.bloom_words_max = std::max(bw, 2048);
-        int occ = 1;
-        (void)rte::search_occupancy(probe, 256, wpb, &occ);
-        const int64_t resident = (int64_t)std::max(1, occ) * cus;  // workgroups
-        if (mult == 0) {
-          t.grid = (int)std::min<int64_t>((nq + wpb - 1) / wpb, 1 << 20);
-        } else if (mult > 0) {
-          t.grid = (int)(resident * mult);
-        } else {
-          const int64_t waves = resident * wpb;
-          const int64_t rounds = (nq + waves - 1) / waves;
-          const int64_t need = (nq + rounds - 1) / rounds;  // waves per round
-          t.grid = (int)std::max<int64_t>(1, (need + wpb - 1) / wpb);
-        }
-        float ms_sum = 0.f;
-        for (int rep = 0; rep < 4 && rc == RTE_OK; ++rep) {
-          (void)hipEventRecord(e0, c->stream);
-          rc = rte_search_batched(c, PQ, nq, q_stride, leaf, k_per_q, beam_per_q, &t, k_stride,
-                                  ids_out, dists_out, RTE_DEVICE);
-          (void)hipEventRecord(e1, c->stream);
-          (void)hipEventSynchronize(e1);
-          float ms = 0.f;
-          (void)hipEventElapsedTime(&ms, e0, e1);
-          if (rep > 0) ms_sum += ms;  // the first run builds the graph
-        }
-        const float ms = ms_sum / 3.f;
-        if (rc == RTE_OK && ms < best_t) {
-          best_t = ms;
-          bp = t;
-        }
+  int first_rc = RTE_OK;
+  std::string first_msg;
+  bool any = false;
+  for (const rte_params& t : cands) {
+    // the first run captures the launch; then three timed runs
+    int rc = rte_search_batched(c, PQ, nq, q_stride, leaf, k_per_q, beam_per_q, &t, k_stride,
+                                ids_out, dists_out, RTE_DEVICE);
+    float ms_sum = 0.f;
+    for (int rep = 0; rep < 3 && rc == RTE_OK; ++rep) {
+      (void)hipEventRecord(e0, c->stream);
+      rc = rte_search_batched(c, PQ, nq, q_stride, leaf, k_per_q, beam_per_q, &t, k_stride,
+                              ids_out, dists_out, RTE_DEVICE);
+      (void)hipEventRecord(e1, c->stream);
+      if (hipEventSynchronize(e1) != hipSuccess && rc == RTE_OK)
+        rc = fail(c, RTE_EHIP, "autotune: event wait failed");
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      ms_sum += ms;
+    }
+    if (rc != RTE_OK) {  // this shape failed: remember why, try the others
+      if (first_rc == RTE_OK) {
+        first_rc = rc;
+        first_msg = c->err;
       }
+      continue;
+    }
+    any = true;
+    const float ms = ms_sum / 3.f;
+    if (ms < best_t) {
+      best_t = ms;
+      *best = t;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (rc != RTE_OK) return rc;
-  *best = bp;
+  if (!any) {
+    c->err = first_msg;
+    return first_rc;
+  }
+  // leave the outputs of the chosen configuration
+  if (int r = rte_search_batched(c, PQ, nq, q_stride, leaf, k_per_q, beam_per_q, best, k_stride,
+                                 ids_out, dists_out, RTE_DEVICE))
+    return r;
   *best_ms = best_t;
   return RTE_OK;
 }
@@ -932,7 +1160,7 @@ void rte_hash_parameter(int nc, int* hash_len, int* bit, int* hash) {
 }
 
 int rte_search_supported(int beam, int batch, int d) {
-  return rte::search_supported(pow2_ceil(std::max(beam, 32)), batch, d);
+  return rte::search_supported(pow2_ceil(std::max(beam, 1)), batch, d);
 }
 
 const char* rte_version(void) { return "rt_entry_amd 0.1 gfx950"; }

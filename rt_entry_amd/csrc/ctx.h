@@ -84,6 +84,9 @@ struct rte_ctx {
   // batched search: per-query shapes, query order, work counter, and the
   // captured launch (reused while its key is unchanged)
   DevBuf b_k, b_beam, b_order, b_counter, b_kin, b_bin;
+  // RTE_ORDER_LEAF: key histogram (zero between launches), scatter cursors,
+  // per-range work counters
+  DevBuf b_hist, b_cursor, b_seg;
   hipGraphExec_t gexec = nullptr;
   std::vector<unsigned char> gkey;
 
@@ -98,3 +101,27 @@ struct rte_ctx {
 
 // record a failure message on the context and return code
 int rte_fail(rte_ctx* c, int code, const char* fmt, ...);
+
+namespace rte {
+// One device array holding every buffer of a batched launch, for calls that
+// must not allocate.  Layout (int32 words): counter block (16 x int64) |
+// range counters (16 x 8 x int64) | key histogram (max_keys) | key cursors
+// (max_keys) | k (max_queries) | beam (max_queries) | order (max_queries).
+constexpr int kScratchCounterWords = 2 * 16 + 2 * 16 * 8;
+struct BatchScratch {
+  int32_t* base;  // device address
+  int64_t max_queries;
+  int max_keys;
+};
+constexpr int64_t batch_scratch_words(int64_t max_queries, int max_keys) {
+  return kScratchCounterWords + 2 * (int64_t)max_keys + 3 * max_queries;
+}
+}  // namespace rte
+
+// rte_search_batched on device buffers with every launch buffer taken from
+// sc: no allocation and no captured graph (direct launches; the buffers that
+// must start at zero are cleared by a kernel first).
+int rte_search_batched_in(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stride,
+                          const int32_t* leaf, const int32_t* k_per_q, const int32_t* beam_per_q,
+                          const rte_params* p, int k_stride, int32_t* ids_out, float* dists_out,
+                          const rte::BatchScratch& sc);

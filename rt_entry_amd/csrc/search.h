@@ -42,17 +42,23 @@ struct SearchArgs {
   const int32_t* beam_per_q;  // uneven batches (or null)
   int64_t* work_counter;      // persistent launch: shared query counter (or null)
   const int32_t* order;       // persistent launch: query order (or null)
+  int segments;               // persistent launch: slot ranges with their own
+                              // counters (work_counter + 16 * range), 0/1: one
+  int wpq;                    // waves per query (1, 2, 4); > 1: one query per
+                              // workgroup of wpq waves
 };
 
 int search_supported(int beam_max, int m, int d);
-// wpb: queries (waves) per workgroup, 1..4
+// wpb: waves per workgroup, 1..4 (one query each), or a.wpq (> 1: one query
+// of a.wpq waves per workgroup)
 hipError_t launch_search(const SearchArgs& a, int beam_max, int grid, int wpb, hipStream_t st);
 // resident workgroups per CU for this launch shape
 hipError_t search_occupancy(const SearchArgs& a, int beam_max, int wpb, int* blocks_per_cu);
 
-// per-beam-register-count launchers (one translation unit each); with occ
-// non-null they report occupancy instead of launching
-template <int NR, int MR>
-hipError_t launch_nm(const SearchArgs& a, int grid, int wpb, hipStream_t st, int* occ);
+// launchers per beam register count, batch register count and waves per
+// query (translation units search_nr*.hip); with occ non-null they report
+// occupancy instead of launching
+template <int NR, int MR, int WPQ>
+hipError_t launch_nmw(const SearchArgs& a, int grid, int wpb, hipStream_t st, int* occ);
 
 }  // namespace rte

@@ -274,74 +274,88 @@ struct WaveLds {
   int* sbi;         // 64*MR sorted batch ids
 };
 
-template <int MR>
-__host__ __device__ constexpr int lds_words_extra() {
-  return 4 * 64 * MR;
-}
 
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
-// Sum of 16 per-point lane partials over the 32 lanes of each half-wave,
-// by recursive halving: at distance o the lane keeps the half of its points
-// selected by lane bit o and adds the partner's partial of those points.
-// Every partial combines lanes l and l^o exactly as the shfl_down tree
-// 16,8,4,2,1 does (graph_search.cuh:873-878), so each total is bit-identical
-// to the reference's; afterwards lanes 2p and 2p+1 of a half hold point p.
-__device__ __forceinline__ float transpose_reduce16(const float (&v)[16], int lane) {
-  float a[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    // x = v[i], y = v[i+8]: x' = [x0 y0 x2 y2], y' = [x1 y1 x3 y3], so
-    // x' + y' is point i in rows 0/2 and point i+8 in rows 1/3
-    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[i]),
-                                                    __float_as_uint(v[i + 8]), false, false);
-    a[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+// One level of the lane-pair sum at distance O inside each 32-lane
+// half-wave: lanes with lane bit O clear return lo + the partner's lo, lanes
+// with it set return hi + the partner's hi (partner = lane ^ O).
+template <int O>
+__device__ __forceinline__ float pair_sum(float lo, float hi, int lane) {
+  if constexpr (O == 16) {
+    // x = lo, y = hi: x' = [x0 y0 x2 y2], y' = [x1 y1 x3 y3] by 16-lane row,
+    // so x' + y' is (own + partner) lo in rows 0/2 and hi in rows 1/3
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(lo), __float_as_uint(hi),
+                                                    false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  } else if constexpr (O == 8) {
+    const float x = lo + dpp_f<0x108>(lo);  // row_shl:8, lane+8
+    const float y = hi + dpp_f<0x118>(hi);  // row_shr:8, lane-8
+    return (lane & 8) ? y : x;
+  } else if constexpr (O == 4) {
+    const float x = lo + dpp_f<0x104>(lo);  // row_shl:4
+    const float y = hi + dpp_f<0x114>(hi);  // row_shr:4
+    return (lane & 4) ? y : x;
+  } else if constexpr (O == 2) {
+    const float x = lo + dpp_f<0x4E>(lo);  // quad_perm xor 2
+    const float y = hi + dpp_f<0x4E>(hi);
+    return (lane & 2) ? y : x;
+  } else {
+    static_assert(O == 1, "pair distance");
+    const float x = lo + dpp_f<0xB1>(lo);  // quad_perm xor 1
+    const float y = hi + dpp_f<0xB1>(hi);
+    return (lane & 1) ? y : x;
   }
-  float b[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const float lo = a[i] + dpp_f<0x108>(a[i]);          // row_shl:8, lane+8
-    const float hi = a[i + 4] + dpp_f<0x118>(a[i + 4]);  // row_shr:8, lane-8
-    b[i] = (lane & 8) ? hi : lo;
-  }
-  float c[2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const float lo = b[i] + dpp_f<0x104>(b[i]);          // row_shl:4
-    const float hi = b[i + 2] + dpp_f<0x114>(b[i + 2]);  // row_shr:4
-    c[i] = (lane & 4) ? hi : lo;
-  }
-  const float lo = c[0] + dpp_f<0x4E>(c[0]);  // quad_perm xor 2
-  const float hi = c[1] + dpp_f<0x4E>(c[1]);
-  const float e = (lane & 2) ? hi : lo;
-  return e + dpp_f<0xB1>(e);  // quad_perm xor 1
 }
 
-// shfl_down tree of one point per half-wave; lane 0 of each half has it
-__device__ __forceinline__ float tree_reduce(float acc, int lane) {
-  acc += xor_lane<16>(acc, lane);
-  acc += xor_lane<8>(acc, lane);
-  acc += xor_lane<4>(acc, lane);
-  acc += xor_lane<2>(acc, lane);
-  acc += xor_lane<1>(acc, lane);
-  return acc;
+// Sum of CNT per-point lane partials over the 32 lanes of each half-wave by
+// recursive halving, distances O = 16, 8, 4, 2, 1 in that order: while a
+// lane still holds several points it keeps the half selected by lane bit O
+// and adds the partner's partials of those points; once it holds one point
+// the remaining levels are plain butterfly sums.  Every partial combines
+// lanes l and l^O at the level the shfl_down tree 16,8,4,2,1 does
+// (graph_search.cuh:873-878), so each total is bit-identical to the
+// reference's.  Afterwards the 2^(5 - log2 CNT) consecutive lanes from
+// (l32 >> (5 - log2 CNT)) << (5 - log2 CNT) hold point l32 >> (5 - log2 CNT).
+template <int CNT, int O>
+__device__ __forceinline__ float treduce(const float (&v)[CNT], int lane) {
+  if constexpr (O == 0) {
+    static_assert(CNT == 1, "more points than lane levels");
+    return v[0];
+  } else if constexpr (CNT == 1) {
+    const float e[1] = {v[0] + xor_lane<O>(v[0], lane)};
+    return treduce<1, O / 2>(e, lane);
+  } else {
+    float nx[CNT / 2];
+#pragma unroll
+    for (int i = 0; i < CNT / 2; ++i) nx[i] = pair_sum<O>(v[i], v[i + CNT / 2], lane);
+    return treduce<CNT / 2, O / 2>(nx, lane);
+  }
 }
 
-template <bool TUNED, int NR, int MR, int J>
+constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
+
+template <bool TUNED, int NR, int MR, int J, int WPQ>
 struct SearchCore {
   // points per half-wave in one load group: all 16 when the vectors fit in
-  // 64 VGPRs, so one iteration's gathers are issued together
-  static constexpr int PB = (64 / J) >= 16 ? 16 : ((64 / J) > 0 ? (64 / J) : 1);
+  // 64 VGPRs, so one iteration's gathers are issued together; with WPQ
+  // waves per query each wave takes 1/WPQ of them
+  static constexpr int PB0 = (64 / J) >= 16 ? 16 : ((64 / J) > 0 ? (64 / J) : 1);
+  static constexpr int PB = PB0 / WPQ > 0 ? PB0 / WPQ : 1;
+  static constexpr int SH = 5 - ilog2c(PB);  // a point's lanes: l32 >> SH
 
   // distances of the valid ids of B (id in [0,np)); others get MAXD.
+  // Every wave of the query compacts the same ids (its own copy); wave `sub`
+  // computes the load groups sub, sub + WPQ, ... into cdist (shared by the
+  // query's waves when WPQ > 1, a workgroup barrier before the reads).
   // Returns the number of distances computed.
   static __device__ __forceinline__ int batch_dists(const SearchArgs& a, const WaveLds& s,
-                                                    const float (&qv)[J], int lane,
-                                                    const int (&bi)[MR], float (&bd)[MR],
-                                                    int m) {
+                                                    float* cdist, const float (&qv)[J],
+                                                    int lane, int sub, const int (&bi)[MR],
+                                                    float (&bd)[MR], int m) {
     const int half = lane >> 5, l32 = lane & 31;
     int total = 0;
     int cix[MR];
@@ -361,7 +375,7 @@ struct SearchCore {
     const bool full = a.d == 32 * J;
     const char* basep = reinterpret_cast<const char*>(a.base) + 4 * l32;
     const int64_t rowb = a.stride * 4;
-    for (int p0 = 0; p0 < total; p0 += 2 * PB) {
+    for (int p0 = 2 * PB * sub; p0 < total; p0 += 2 * PB * WPQ) {
       float v[PB][J];
 #pragma unroll
       for (int pb = 0; pb < PB; ++pb) {
@@ -387,22 +401,17 @@ struct SearchCore {
         }
         part[pb] = acc;
       }
-      if constexpr (PB == 16) {
-        const float tot = transpose_reduce16(part, lane);
-        const int kk = p0 + 2 * (l32 >> 1) + half;
-        if ((lane & 1) == 0 && kk < total) s.cdist[kk] = tot;
-      } else {
-#pragma unroll
-        for (int pb = 0; pb < PB; ++pb) {
-          const float tot = tree_reduce(part[pb], lane);
-          const int kk = p0 + 2 * pb + half;
-          if (l32 == 0 && kk < total) s.cdist[kk] = tot;
-        }
-      }
+      const float tot = treduce<PB, 16>(part, lane);
+      const int kk = p0 + 2 * (l32 >> SH) + half;
+      if ((l32 & ((1 << SH) - 1)) == 0 && kk < total) cdist[kk] = tot;
     }
-    wave_sync();
+    if constexpr (WPQ > 1) {
+      __syncthreads();  // the query's waves are the workgroup
+    } else {
+      wave_sync();
+    }
 #pragma unroll
-    for (int r = 0; r < MR; ++r) bd[r] = cix[r] >= 0 ? s.cdist[cix[r]] : kMaxD;
+    for (int r = 0; r < MR; ++r) bd[r] = cix[r] >= 0 ? cdist[cix[r]] : kMaxD;
     wave_sync();
     return total;
   }
@@ -466,49 +475,125 @@ __device__ __forceinline__ uint32_t scan_chunk(const uint64_t (&bf)[NR], const u
   return vf | zm;
 }
 
-// Next query of this wave.  Without a work counter: a grid stride.  With one
-// (persistent launch): the first query of every wave is its global wave id
-// (no atomic), later ones come from the counter, which counts past the
-// statically assigned prefix and is zero again when the launch ends; either
-// way through the order array when there is one.
-__device__ __forceinline__ int64_t next_query(const SearchArgs& a, int64_t prev_slot, int wave,
-                                              int lane) {
-  const int wpb = blockDim.x >> 6;
-  const int64_t waves = (int64_t)gridDim.x * wpb;
-  int64_t slot;
-  if (!a.work_counter) {
-    slot = prev_slot < 0 ? (int64_t)blockIdx.x * wpb + wave : prev_slot + waves;
-  } else if (prev_slot < 0) {
-    slot = (int64_t)blockIdx.x * wpb + wave;
-  } else if (waves >= a.nq) {
-    slot = a.nq;  // every query was assigned statically
-  } else {
-    unsigned long long* ctr = reinterpret_cast<unsigned long long*>(a.work_counter);
-    unsigned long long i = 0;
-    if (lane == 0) {
-      i = atomicAdd(ctr, 1ull);
-      // a launch makes nq - waves increments that get a query and one per
-      // wave that does not: nq in all, so the one returning nq - 1 is the
-      // last, and it clears the counter for the next launch
-      if (i == (unsigned long long)(a.nq - 1)) atomicExch(ctr, 0ull);
-    }
-    slot = waves + (int64_t)__builtin_amdgcn_readfirstlane((int)i);
+// One scalar fetch from a device-wide counter by a query unit: lane 0 of the
+// unit's first wave increments, every wave of the unit gets the value (a
+// workgroup barrier and an LDS slot when the unit is several waves).
+template <int WPQ>
+__device__ __forceinline__ unsigned long long unit_fetch(unsigned long long* ctr, int wave,
+                                                         int lane, int64_t* bcast,
+                                                         unsigned long long clear_at) {
+  unsigned long long i = 0;
+  if (wave % WPQ == 0 && lane == 0) {
+    i = atomicAdd(ctr, 1ull);
+    if (i == clear_at) atomicExch(ctr, 0ull);
+    if constexpr (WPQ > 1) *bcast = (int64_t)i;
   }
-  return slot;
+  if constexpr (WPQ > 1) {
+    __syncthreads();
+    i = (unsigned long long)*bcast;
+    __syncthreads();  // the slot is rewritten only after every wave read it
+    return i;
+  } else {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)i);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(i >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+  }
+}
+
+// Next query slot of this unit (a wave, or the WPQ waves of a workgroup that
+// share one query).  Without a work counter: a grid stride.  With one
+// (persistent launch): the first slot of every unit is its global unit id
+// (no atomic), later ones come from the counter, which counts past the
+// statically assigned prefix and is zero again when the launch ends.  With
+// `segments` > 1 the slots are cut into that many contiguous ranges, one per
+// group of workgroups blockIdx % segments (the workgroups of one XCD under
+// round-robin dispatch), each range with its own counter (reset before the
+// launch); a unit whose range is exhausted takes slots from the next ranges.
+// Slots map to queries through the order array when there is one.
+template <int WPQ>
+__device__ __forceinline__ int64_t next_query(const SearchArgs& a, int64_t prev_slot, int wave,
+                                              int lane, int64_t* bcast) {
+  const int qpb = (blockDim.x >> 6) / WPQ;  // queries per workgroup
+  const int64_t units = (int64_t)gridDim.x * qpb;
+  const int qw = wave / WPQ;
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(a.work_counter);
+  if (!ctr) return prev_slot < 0 ? (int64_t)blockIdx.x * qpb + qw : prev_slot + units;
+  if (a.segments <= 1) {
+    if (prev_slot < 0) return (int64_t)blockIdx.x * qpb + qw;
+    if (units >= a.nq) return a.nq;  // every query was assigned statically
+    // a launch makes nq - units increments that get a query and one per
+    // unit that does not: nq in all, so the one returning nq - 1 is the
+    // last, and it clears the counter for the next launch
+    return units + (int64_t)unit_fetch<WPQ>(ctr, wave, lane, bcast,
+                                            (unsigned long long)(a.nq - 1));
+  }
+  const int S = a.segments;
+  const int x = (int)(blockIdx.x % (unsigned)S);
+  if (prev_slot < 0) {
+    const int64_t lu = (int64_t)(blockIdx.x / (unsigned)S) * qpb + qw;
+    const int64_t b = a.nq * x / S, e = a.nq * (x + 1) / S;
+    if (b + lu < e) return b + lu;
+  }
+  for (int t = 0; t < S; ++t) {
+    const int y = (x + t) % S;
+    const int64_t b = a.nq * y / S, e = a.nq * (y + 1) / S;
+    // units statically given a slot of range y: one per workgroup of group y
+    // (signed: a grid smaller than S leaves groups without workgroups)
+    const int64_t uy = (((int64_t)gridDim.x - y + S - 1) / S) * qpb;
+    if (uy >= e - b) continue;
+    const int64_t i = (int64_t)unit_fetch<WPQ>(ctr + 16 * y, wave, lane, bcast, ~0ull);
+    if (b + uy + i < e) return b + uy + i;
+  }
+  return a.nq;
 }
 
 __device__ __forceinline__ int64_t query_of(const SearchArgs& a, int64_t slot) {
   return a.order ? a.order[slot] : slot;
 }
 
-template <bool TUNED, int NR, int MR, int J>
+// Distance buffer of the next batch: the wave's own, or one of the query's
+// two shared buffers, alternating per batch so a wave that runs ahead into
+// the next batch never writes the buffer another wave is still reading
+// (between two uses of one buffer every wave passes a barrier after its
+// reads).
+template <int MR, int WPQ>
+__device__ __forceinline__ float* dist_buf_t(const WaveLds& s, uint32_t* qshared, int& parity) {
+  if constexpr (WPQ == 1) {
+    return s.cdist;
+  } else {
+    float* p = reinterpret_cast<float*>(qshared) + (parity & 1) * 64 * MR;
+    parity ^= 1;
+    return p;
+  }
+}
+#define dist_buf(s, q, par) dist_buf_t<MR, WPQ>(s, q, par)
+
+// LDS words of one wave's private region past the filter: compacted ids,
+// sorted batch (ids, dists) and, for one wave per query, the distances
+template <int MR, int WPQ>
+__host__ __device__ constexpr int lds_words_wave() {
+  return (WPQ > 1 ? 3 : 4) * 64 * MR;
+}
+// LDS words a query shares between its waves (WPQ > 1): two distance
+// buffers (alternating batches) and the broadcast slot of the work queue
+template <int MR, int WPQ>
+__host__ __device__ constexpr int lds_words_query() {
+  return WPQ > 1 ? 2 * 64 * MR + 4 : 0;
+}
+
+template <bool TUNED, int NR, int MR, int J, int WPQ>
 __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
-  using Core = SearchCore<TUNED, NR, MR, J>;
+  using Core = SearchCore<TUNED, NR, MR, J, WPQ>;
   extern __shared__ uint32_t smem[];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int region = a.bloom_words_max + lds_words_extra<MR>();
+  const int sub = wave % WPQ;  // this wave's share of the query's distances
+  const int region = a.bloom_words_max + lds_words_wave<MR, WPQ>();
   uint32_t* base_lds = smem + wave * region;
+  // shared by the query's waves (WPQ > 1; one query per workgroup)
+  uint32_t* qshared = smem + (blockDim.x >> 6) * region;
+  int64_t* bcast = reinterpret_cast<int64_t*>(qshared + 2 * 64 * MR);
+  int parity = 0;  // alternates the shared distance buffers
   const int shift = a.shift;
   const int D = 1 << shift;
   const int W = a.width;
@@ -521,8 +606,8 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
   const int beam_all = __builtin_amdgcn_readfirstlane(a.beam);
   const int k_all = __builtin_amdgcn_readfirstlane(a.k);
 
-  for (int64_t slot = next_query(a, -1, wave, lane); slot < a.nq;
-       slot = next_query(a, slot, wave, lane)) {
+  for (int64_t slot = next_query<WPQ>(a, -1, wave, lane, bcast); slot < a.nq;
+       slot = next_query<WPQ>(a, slot, wave, lane, bcast)) {
     const int64_t q = query_of(a, slot);
     // per-query shape (uneven batches carry their own k and beam)
     int n = beam_all, k = k_all;
@@ -551,9 +636,9 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
     WaveLds s;
     s.bloom = base_lds;
     s.cid = (int*)(base_lds + a.bloom_words_max);
-    s.cdist = (float*)(s.cid + 64 * MR);
-    s.sbd = s.cdist + 64 * MR;
+    s.sbd = (float*)(s.cid + 64 * MR);
     s.sbi = (int*)(s.sbd + 64 * MR);
+    s.cdist = WPQ > 1 ? nullptr : (float*)(s.sbi + 64 * MR);
     (void)bhash;  // fixed per mode (kHashes), checked on the host
     Bloom<kHashes<TUNED>> bl{s.bloom, (uint32_t)bwords, (uint32_t)(31 - __clz(bwords)),
                              (uint32_t)bwords * 32u};
@@ -605,7 +690,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
         bi[r] = id;
       }
       wave_sync();
-      nE += Core::batch_dists(a, s, qv, lane, bi, bd, m);
+      nE += Core::batch_dists(a, s, dist_buf(s, qshared, parity), qv, lane, sub, bi, bd, m);
       Core::merge_batch(s, lane, ld, li, bd, bi, n, m, false);
     }
 
@@ -688,7 +773,8 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
         }
         wave_sync();
       }
-      nE += Core::batch_dists(a, s, qv, lane, bi, bd, cnt << shift);
+      nE += Core::batch_dists(a, s, dist_buf(s, qshared, parity), qv, lane, sub, bi, bd,
+                              cnt << shift);
       Core::merge_batch(s, lane, ld, li, bd, bi, n, m, true);
       // rescan (:1495-1516)
       {
@@ -720,12 +806,12 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       const int sl = r * 64 + lane;
-      if (sl < k) {
+      if (sub == 0 && sl < k) {
         a.ids_out[ob + sl] = li[r] < 0 ? -li[r] : li[r];
         if (a.dists_out) a.dists_out[ob + sl] = ld[r];
       }
     }
-    if (a.stats_out && lane == 0) {
+    if (a.stats_out && sub == 0 && lane == 0) {
       a.stats_out[q * 3 + 0] = nE;
       a.stats_out[q * 3 + 1] = nX;
       a.stats_out[q * 3 + 2] = iter;
@@ -734,33 +820,40 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
   }
 }
 
-template <bool TUNED, int NR, int MR, int J>
+template <int MR, int WPQ>
+size_t search_lds_bytes(int bloom_words_max, int wpb) {
+  return ((size_t)wpb * (bloom_words_max + lds_words_wave<MR, WPQ>()) +
+          lds_words_query<MR, WPQ>()) * 4;
+}
+
+template <bool TUNED, int NR, int MR, int J, int WPQ>
 hipError_t launch_t(const SearchArgs& a, int grid, int wpb, hipStream_t st, int* occ) {
-  const size_t lds = (size_t)wpb * (a.bloom_words_max + lds_words_extra<MR>()) * 4;
+  const size_t lds = search_lds_bytes<MR, WPQ>(a.bloom_words_max, wpb);
   if (occ) {  // resident workgroups per CU for this shape
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, search_kernel<TUNED, NR, MR, J>,
-                                                        64 * wpb, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        occ, search_kernel<TUNED, NR, MR, J, WPQ>, 64 * wpb, lds);
   }
-  hipLaunchKernelGGL((search_kernel<TUNED, NR, MR, J>), dim3(grid), dim3(64 * wpb), lds, st, a);
+  hipLaunchKernelGGL((search_kernel<TUNED, NR, MR, J, WPQ>), dim3(grid), dim3(64 * wpb), lds, st,
+                     a);
   return hipGetLastError();
 }
 
-template <int NR, int MR>
-hipError_t launch_nm(const SearchArgs& a, int grid, int wpb, hipStream_t st, int* occ) {
+template <int NR, int MR, int WPQ>
+hipError_t launch_nmw(const SearchArgs& a, int grid, int wpb, hipStream_t st, int* occ) {
   const int J = (a.d + 31) / 32;
   const bool tuned = a.mode != 0;
-#define RTE_J_CASES(T)                                                 \
-  switch (J) {                                                         \
-    case 1:                                                            \
-    case 2: return launch_t<T, NR, MR, 2>(a, grid, wpb, st, occ);      \
-    case 3: return launch_t<T, NR, MR, 3>(a, grid, wpb, st, occ);      \
-    case 4: return launch_t<T, NR, MR, 4>(a, grid, wpb, st, occ);      \
-    case 5:                                                            \
-    case 6:                                                            \
-    case 7:                                                            \
-    case 8: return launch_t<T, NR, MR, 8>(a, grid, wpb, st, occ);      \
-    case 30: return launch_t<T, NR, MR, 30>(a, grid, wpb, st, occ);    \
-    default: return hipErrorInvalidValue;                              \
+#define RTE_J_CASES(T)                                                      \
+  switch (J) {                                                              \
+    case 1:                                                                 \
+    case 2: return launch_t<T, NR, MR, 2, WPQ>(a, grid, wpb, st, occ);      \
+    case 3: return launch_t<T, NR, MR, 3, WPQ>(a, grid, wpb, st, occ);      \
+    case 4: return launch_t<T, NR, MR, 4, WPQ>(a, grid, wpb, st, occ);      \
+    case 5:                                                                 \
+    case 6:                                                                 \
+    case 7:                                                                 \
+    case 8: return launch_t<T, NR, MR, 8, WPQ>(a, grid, wpb, st, occ);      \
+    case 30: return launch_t<T, NR, MR, 30, WPQ>(a, grid, wpb, st, occ);    \
+    default: return hipErrorInvalidValue;                                   \
   }
   if (tuned) {
     RTE_J_CASES(true)

@@ -1,0 +1,10 @@
+// Search kernel instantiations for beams up to 64 with two or four waves
+// per query.
+#include "search_impl.h"
+
+namespace rte {
+template hipError_t launch_nmw<1, 1, 2>(const SearchArgs&, int, int, hipStream_t, int*);
+template hipError_t launch_nmw<1, 2, 2>(const SearchArgs&, int, int, hipStream_t, int*);
+template hipError_t launch_nmw<1, 1, 4>(const SearchArgs&, int, int, hipStream_t, int*);
+template hipError_t launch_nmw<1, 2, 4>(const SearchArgs&, int, int, hipStream_t, int*);
+}  // namespace rte

@@ -1,10 +1,9 @@
-// Search kernel instantiations for beams up to 256 (one translation unit per
-// beam register count, so the build compiles them in parallel).
+// Search kernel instantiations for beams up to 256, one wave per query (one
+// translation unit per beam register count, so the build compiles them in
+// parallel).
 #include "search_impl.h"
 
 namespace rte {
-template hipError_t launch_nm<4, 1>(const SearchArgs&, int, int, hipStream_t,
-                                           int*);
-template hipError_t launch_nm<4, 2>(const SearchArgs&, int, int, hipStream_t,
-                                           int*);
+template hipError_t launch_nmw<4, 1, 1>(const SearchArgs&, int, int, hipStream_t, int*);
+template hipError_t launch_nmw<4, 2, 1>(const SearchArgs&, int, int, hipStream_t, int*);
 }  // namespace rte
