@@ -237,6 +237,27 @@ int rte_search_batched(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stri
                        const int32_t* beam_per_q, const rte_params* p, int k_stride,
                        int32_t* ids_out, float* dists_out, int where);
 
+/* rte_search_batched without any allocation (no reference counterpart; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * semantics and results as rte_search_batched, with every pointer a device
+ * pointer; the kernels are launched directly on the context's stream (no
+ * captured graph), and their work counters, per-query k/beam, query order
+ * and leaf-sort buffers live in one static device array per GPU (3.8 MB,
+ * held until the process ends), cleared by a kernel at the start of each
+ * call.  Limits: nq <= RTE_NOALLOC_MAX_QUERIES, and with
+ * RTE_ORDER_LEAF n_leaf (x 5 for an uneven batch) <= RTE_NOALLOC_MAX_KEYS;
+ * beyond them the call fails with RTE_EINVAL.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams, other
+ * contexts, other threads) race and must be ordered by the caller.  The
+ * rte_set_* calls that set the context up still allocate; make them first. */
+#define RTE_NOALLOC_MAX_QUERIES 262144
+#define RTE_NOALLOC_MAX_KEYS 81920
+int rte_search_batched_noalloc(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stride,
+                               const int32_t* leaf, const int32_t* k_per_q,
+                               const int32_t* beam_per_q, const rte_params* p, int k_stride,
+                               int32_t* ids_out, float* dists_out);
+
 /* Autotuner for rte_search_batched: times the batch (device buffers) under
  * every launch shape -- wavefronts per query (wpq 1, 2, 4) x queries per
  * workgroup (wg_waves 1, 2, 4 at wpq 1) x grid (RTE_GRID_PER_QUERY,

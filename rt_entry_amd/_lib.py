@@ -24,7 +24,7 @@ EXPORTS = [
     "rte_create", "rte_destroy", "rte_last_error", "rte_set_stream", "rte_get_stream",
     "rte_synchronize", "rte_set_base", "rte_set_graph", "rte_set_pca", "rte_project",
     "rte_set_leaves", "rte_build_kdtree", "rte_expand_aabbs", "rte_select_entry",
-    "rte_search", "rte_search_batched", "rte_autotune_batched",
+    "rte_search", "rte_search_batched", "rte_search_batched_noalloc", "rte_autotune_batched",
     "rte_query", "rte_mlp_create", "rte_mlp_train_step",
     "rte_mlp_predict", "rte_predict_entry", "rte_set_rerank_base", "rte_rerank", "rte_knn", "rte_knn_rows", "rte_graph_from_knn", "rte_pca_fit", "rte_pca_rotate", "rte_build_kdtree_gpu", "rte_mlp_get", "rte_mlp_set", "rte_mlp_grad", "rte_mlp_set_input_norm", "rte_hash_parameter",
     "rte_search_supported", "rte_version",
@@ -76,6 +76,8 @@ def lib():
         "rte_search": ([P, P, i64, i64, P, C.POINTER(Params), P, P, P, i32], i32),
         "rte_search_batched": ([P, P, i64, i64, P, P, P, C.POINTER(Params), i32, P, P, i32],
                                i32),
+        "rte_search_batched_noalloc": ([P, P, i64, i64, P, P, P, C.POINTER(Params), i32, P, P],
+                                       i32),
         "rte_query": ([P, P, i64, C.POINTER(Params), P, P, P, P], i32),
         "rte_autotune_batched": ([P, P, i64, i64, P, P, P, C.POINTER(Params), i32, P, P, P, i32,
                                   C.POINTER(Params), C.POINTER(C.c_float)], i32),

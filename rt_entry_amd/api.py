@@ -258,6 +258,24 @@ class Context:
                                            int(k_stride), pi, pd, w))
         return ids, dists
 
+    def search_batched_noalloc(self, PQ, leaf, k_per_q, beam_per_q, params: Params, k_stride,
+                               out):
+        """rte_search_batched_noalloc: the batched search with no allocation
+        (device tensors only, outputs given)."""
+        ids, dists = out
+        P = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        nq, qs = PQ.shape
+        if (k_per_q is None) != (beam_per_q is None):
+            raise ValueError("k_per_q and beam_per_q: both or neither")
+        params = Params(*[getattr(params, f) for f, _ in Params._fields_])
+        if beam_per_q is not None:  # the batch's beam bound, as search_batched
+            bound = max(int(beam_per_q.max()), int(k_per_q.max()), 32)
+            params.beam = 1 << (bound - 1).bit_length()
+        self._chk(lib().rte_search_batched_noalloc(self._h, P(PQ), int(nq), int(qs), P(leaf),
+                                                   P(k_per_q), P(beam_per_q), C.byref(params),
+                                                   int(k_stride), P(ids), P(dists)))
+        return ids, dists
+
     # ---- learned entry predictor (device tensors)
     def mlp_create(self, d_in: int, h1: int, h2: int, n_out: int, seed: int = 1):
         self._chk(lib().rte_mlp_create(self._h, int(d_in), int(h1), int(h2), int(n_out),
