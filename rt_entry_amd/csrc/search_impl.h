@@ -416,11 +416,49 @@ struct SearchCore {
     return total;
   }
 
+  // True when no batch element can enter the beam, so the step below would
+  // leave L as it is.  The half-cleaner cascade that produced L leaves every
+  // comparator relation of all its stages true (its first stage puts each
+  // lower element at or below its partner, and comparator networks keep
+  // element-wise order, so the recursion keeps it), hence a second pass
+  // swaps nothing and L[n-1] is the beam's largest key.  Every element of
+  // B not below L[n-1] therefore loses its min-exchange, and with no swap the
+  // merge is that second pass.  Distances are non-negative floats, so their
+  // bit patterns order as unsigned integers.
+  static __device__ __forceinline__ bool batch_cannot_enter(const float (&ld)[NR],
+                                                            const float (&bd)[MR], int n,
+                                                            int lane) {
+    int mn = -1;  // 0xffffffff: no valid element
+#pragma unroll
+    for (int r = 0; r < MR; ++r) {
+      const uint32_t u = bd[r] != kMaxD ? __float_as_uint(bd[r]) : 0xffffffffu;
+      mn = (int)min((uint32_t)mn, u);
+    }
+    mn = (int)min((uint32_t)mn, (uint32_t)xor_lane<1>(mn, lane));
+    mn = (int)min((uint32_t)mn, (uint32_t)xor_lane<2>(mn, lane));
+    mn = (int)min((uint32_t)mn, (uint32_t)xor_lane<4>(mn, lane));
+    mn = (int)min((uint32_t)mn, (uint32_t)xor_lane<8>(mn, lane));
+    mn = (int)min((uint32_t)mn, (uint32_t)xor_lane<16>(mn, lane));
+    mn = (int)min((uint32_t)mn, (uint32_t)xor_lane<32>(mn, lane));
+    const uint32_t bmin = (uint32_t)__builtin_amdgcn_readfirstlane(mn);
+    if (bmin == 0xffffffffu) return true;
+    const int rl = (n - 1) >> 6, ll = (n - 1) & 63;
+    int lmax = 0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r == rl) lmax = __builtin_amdgcn_readlane(__float_as_int(ld[r]), ll);
+    // REF swaps on a strictly larger distance; TUNED also on an equal
+    // distance with a larger id, so only a strictly smaller L[n-1] is safe
+    return TUNED ? __int_as_float(lmax) < __uint_as_float(bmin)
+                 : __int_as_float(lmax) <= __uint_as_float(bmin);
+  }
+
   // sort B descending, min-exchange into L, merge L ascending
   static __device__ __forceinline__ void merge_batch(const WaveLds& s, int lane, float (&ld)[NR],
                                                      int (&li)[NR], float (&bd)[MR],
                                                      int (&bi)[MR], int n, int m,
                                                      bool skip_max) {
+    if (batch_cannot_enter(ld, bd, n, lane)) return;
     sort_desc<TUNED, MR>(bd, bi, m, lane);
 #pragma unroll
     for (int r = 0; r < MR; ++r) {

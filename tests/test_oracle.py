@@ -384,6 +384,45 @@ def test_merge_batch_wide_batch_keeps_smallest_sorted(mode):
                 assert sorted(zip(np.r_[Ld, Bd].tolist(), np.r_[Li, Bi].tolist())) == allp
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_merge_cascade_idempotent_and_batch_skip(mode):
+    """What lets the GPU kernel skip a whole batch step: the half-cleaner
+    cascade (graph_search.cuh:933-958) is idempotent on any input, bitonic
+    or not, ties included, and its output's last slot holds the largest key;
+    so a batch whose valid elements are all at or above that key (strictly
+    above in the (dist, id) mode) leaves the beam unchanged through sort,
+    min-exchange and merge.  Every permutation for N <= 4, 1500 of those of
+    N = 8, random inputs with ties up to N = 256."""
+    import itertools
+    rng = np.random.default_rng(19)
+
+    def key(d, i):
+        return (d, abs(i)) if mode == 1 else (d,)
+
+    cases = [np.array(p, np.float32) for n in (2, 4) for p in itertools.permutations(range(n))]
+    perms8 = list(itertools.permutations(range(8)))
+    cases += [np.array(perms8[i], np.float32) for i in rng.choice(len(perms8), 1500, replace=False)]
+    for n in (16, 32, 64, 128, 256):
+        cases += [rng.integers(0, 3 * n, n).astype(np.float32) for _ in range(40)]
+    for x in cases:
+        n = x.size
+        ids = rng.permutation(10 * n)[:n].astype(np.int32) * rng.choice([-1, 1], n).astype(np.int32)
+        ld, li = O.merge_asc(x, ids, mode)
+        d2, i2 = O.merge_asc(ld, li, mode)
+        assert_same(d2, ld, "idempotent dists")
+        assert_same(i2, li, "idempotent ids")
+        top = max(key(a, b) for a, b in zip(ld.tolist(), li.tolist()))
+        assert key(float(ld[-1]), int(li[-1])) == top, "last slot holds the largest key"
+        for m in ((n,) if n <= 8 else (n, 32, 64)):
+            bd = (ld[-1] + rng.integers(0 if mode == 0 else 1, 50, m)).astype(np.float32)
+            bd[rng.random(m) < 0.3] = O.MAXD
+            bi = rng.integers(0, 10 * n, m).astype(np.int32)
+            for skip in (False, True):
+                Ld, Li, _, _ = O.merge_batch(ld, li, bd, bi, mode, skip_max=skip)
+                assert_same(Ld, ld, f"batch above the beam n={n} m={m}")
+                assert_same(Li, li, f"batch above the beam n={n} m={m}")
+
+
 def test_search_max_iter_zero_returns_sorted_entries():
     rng = np.random.default_rng(7)
     n, d, cap = 200, 32, 64
