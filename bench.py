@@ -53,6 +53,11 @@ def kernel_source_id() -> str:
     return h.hexdigest()[:16]
 
 
+# entry-space links per workload (per-row count, leaf window): the 10k-cluster
+# workload needs them to reach recall 0.95 (its kNN graph has no edges
+# between clusters); the others use the plain kNN + reverse graph
+ENTRY_LINKS_DEFAULT = {5: (16, 0)}
+
 GRID_CODES = {0: "resident", -1: "2x resident", -2: "balanced rounds", -3: "one per query"}
 
 
@@ -67,7 +72,8 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def prepare(workload: int, scale: float, dev, n_train: int = 0, keep_raw: bool = False):
+def prepare(workload: int, scale: float, dev, n_train: int = 0, keep_raw: bool = False,
+            entry_links: tuple = (0, 0)):
     import torch
 
     from rt_entry_amd import data as D
@@ -126,11 +132,23 @@ def prepare(workload: int, scale: float, dev, n_train: int = 0, keep_raw: bool =
     from rt_entry_amd import expand_aabbs
     aabbs = expand_aabbs(boxes, 0.2)  # aabb.cu:27-38, RT_Entry::BlockUp's ratio
     t_kd = time.time() - t0
+    graph_info = dict(seconds=round(t_graph, 2), how=how)
+    if entry_links[0]:
+        # the last r slots of every row: points of the leaves within the
+        # window around the row's own leaf (entry-space links)
+        t0 = time.time()
+        with Context(0) as cx:
+            cx.set_leaves(aabbs, pids)
+            cx.graph_add_entry_links(graph, entry_links[0], window=entry_links[1], seed=1)
+        torch.cuda.synchronize()
+        graph_info["entry_links"] = dict(per_row=entry_links[0], leaf_window=entry_links[1],
+                                         seconds=round(time.time() - t0, 3))
+        log(f"entry links r={entry_links[0]} window={entry_links[1]} in {time.time() - t0:.2f}s")
     pb_h = pbase.cpu().numpy()
     log(f"kd-tree {boxes.shape[0]} leaves cap {cap} in {t_kd:.3f}s")
     out = dict(base=pbase, base_h=pb_h, graph=graph, Q=Q, R=R, mu=mu, gt=gt, aabbs=aabbs,
                pids=pids, n=n, d=d, cap=cap, n_leaf=boxes.shape[0], cfg=cfg,
-               graph_build=dict(seconds=round(t_graph, 2), how=how),
+               graph_build=graph_info,
                prep_seconds=dict(pca_fit_and_rotate=round(t_pca, 3), kdtree=round(t_kd, 4)))
     if n_train:
         t0 = time.time()
@@ -347,6 +365,103 @@ def bench_uneven(args):
     print(json.dumps(out))
 
 
+LAUNCH_VARIANTS = (  # (wpq, wg_waves, order, grid code)
+    (1, 4, 0, 0), (1, 4, 1, 0), (1, 2, 0, 0), (1, 2, 1, 0), (1, 1, 1, 0), (1, 4, 0, -2),
+    (2, 2, 0, 0), (2, 2, 1, 0), (2, 2, 0, -3), (4, 4, 0, 0), (4, 4, 0, -3))
+
+
+def launch_table(args):
+    """Every launch variant of the search on one workload, kd-tree entries:
+    the plain rte_search launch, the batched launch for each (waves per
+    query, queries per workgroup, query order, grid code), and the
+    autotuner's choice; each timed with HIP events on the library's stream
+    over --steps launches after one untimed launch, ids checked equal to the
+    plain launch's.  One JSON line per launch; --nq takes the first n
+    queries (small batches).  Workload 3 times the uneven batch instead."""
+    import torch
+
+    from rt_entry_amd import Context, make_params
+    from rt_entry_amd import data as D
+    dev = torch.device("cuda:0")
+    W = prepare(args.workload, args.scale, dev)
+    nq = min(args.nq or W["Q"].shape[0], W["Q"].shape[0])
+    d = W["d"]
+    ctx = Context(0)
+    st = torch.cuda.Stream(device=dev)
+    ctx.set_stream(st.cuda_stream)
+    torch.cuda.set_stream(st)
+    ctx.set_base(W["base"])
+    ctx.set_graph(W["graph"])
+    ctx.set_leaves(W["aabbs"], W["pids"])
+    ctx.set_pca(W["R"], W["mu"], d)
+    pq = ctx.project(W["Q"][:nq].contiguous())
+    hits = ctx.select_entry(pq)
+    Dg = 1 << (W["cfg"]["degree"] - 1).bit_length()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(args.steps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    def emit(row, alg, ms):
+        row.update(ms=round(ms, 4), achieved_gbs=round(alg / ms / 1e6, 1),
+                   frac=round(alg / (ms * 1e-3) / HBM_PEAK, 4))
+        print(json.dumps(row), flush=True)
+
+    if args.workload == 3:
+        k_q, b_q = D.k_beam_assignment(3, nq)
+        kmax = int(k_q.max())
+        kd_t, bd_t = torch.from_numpy(k_q).to(dev), torch.from_numpy(b_q).to(dev)
+        ids = torch.empty((nq, kmax), dtype=torch.int32, device=dev)
+        dists = torch.empty((nq, kmax), dtype=torch.float32, device=dev)
+        alg = 0.0  # from per-class plain searches with statistics
+        for kk in sorted(set(k_q.tolist())):
+            for bb in sorted(set(b_q.tolist())):
+                sel = np.nonzero((k_q == kk) & (b_q == bb))[0]
+                if sel.size == 0:
+                    continue
+                s_t = torch.from_numpy(sel).to(dev)
+                p = make_params(k=kk, beam=bb, max_iter=args.max_iter or 100, mode=args.mode)
+                _, _, stt = ctx.search(pq[s_t].contiguous(), hits[s_t].contiguous(), p,
+                                       with_stats=True)
+                stt = stt.cpu().numpy().astype(np.int64)
+                alg += float((4 * d * (stt[:, 0] + 1) + 4 * Dg * stt[:, 1] + 4 * bb
+                              + 8 * kk).sum())
+        for wpq, wg, order, grid in LAUNCH_VARIANTS:
+            p = make_params(k=kmax, beam=256, max_iter=args.max_iter or 100, mode=args.mode,
+                            wpq=wpq, wg_waves=wg, order=order, grid=grid)
+            ms = timed(lambda: ctx.search_batched(pq, hits, kd_t, bd_t, p, k_stride=kmax,
+                                                  out=(ids, dists)))
+            emit(dict(workload=3, nq=nq, launch=launch_desc(p)), alg, ms)
+        return
+    settings = [tuple(int(x) for x in args.kd.split(","))] if args.kd else [(64, 33), (256, 51)]
+    for beam, mi in settings:
+        p = make_params(k=10, beam=beam, max_iter=mi, mode=args.mode)
+        ids, dd, stt = ctx.search(pq, hits, p, with_stats=True)
+        stt = stt.cpu().numpy().astype(np.int64)
+        alg = float((4 * d * (stt[:, 0] + 1) + 4 * Dg * stt[:, 1] + 4 * beam + 80).sum())
+        ms = timed(lambda: ctx.search(pq, hits, p, out=(ids, dd, None)))
+        emit(dict(workload=args.workload, nq=nq, beam=beam, max_iter=mi, launch="grid",
+                  recall=round(D.recall(ids.cpu().numpy(), W["gt"][:nq], 10), 4)), alg, ms)
+        i2, d2 = torch.empty_like(ids), torch.empty_like(dd)
+        for wpq, wg, order, grid in LAUNCH_VARIANTS:
+            pb = make_params(k=10, beam=beam, max_iter=mi, mode=args.mode, wpq=wpq, wg_waves=wg,
+                             order=order, grid=grid)
+            ms = timed(lambda: ctx.search_batched(pq, hits, None, None, pb, k_stride=10,
+                                                  out=(i2, d2)))
+            emit(dict(workload=args.workload, nq=nq, beam=beam, max_iter=mi,
+                      launch=launch_desc(pb), ids_equal=bool(torch.equal(ids, i2))), alg, ms)
+        best, tms = ctx.autotune_batched(pq, hits, None, None, p, k_stride=10)
+        print(json.dumps(dict(workload=args.workload, nq=nq, beam=beam, max_iter=mi,
+                              autotuned=launch_desc(best), ms=round(tms, 4))), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -386,6 +501,14 @@ def main():
     ap.add_argument("--min-conf", type=float, default=None,
                     help="kd-tree fallback below this confidence (default: fitted on "
                          "held-out queries from 0.1, 0.2, 0.3, 0.5)")
+    ap.add_argument("--entry-links", default="",
+                    help="R,W: replace the last R slots of every graph row with points of the "
+                         "leaves within W of the row's leaf (default 0,0; workload 5: see "
+                         "ENTRY_LINKS_DEFAULT)")
+    ap.add_argument("--launch-table", action="store_true",
+                    help="time every launch variant (waves per query, workgroup, order, grid) "
+                         "of the search with kd-tree entries instead of the benchmark")
+    ap.add_argument("--nq", type=int, default=0, help="--launch-table: first n queries only")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
 
@@ -394,6 +517,9 @@ def main():
                           "would shard across GPUs with no exchange: each GPU holds the "
                           "whole index and searches its share of the batch)"}))
         sys.exit(2)
+    if args.launch_table:
+        launch_table(args)
+        return
     if args.workload == 3:  # the uneven-batch workload has its own measurement
         bench_uneven(args)
         return
@@ -405,7 +531,10 @@ def main():
     from rt_entry_amd import predictor as P
 
     dev = torch.device("cuda:0")
-    W = prepare(args.workload, args.scale, dev, args.n_train, keep_raw=args.search_dims > 0)
+    links = (tuple(int(x) for x in args.entry_links.split(",")) if args.entry_links
+             else ENTRY_LINKS_DEFAULT.get(args.workload, (0, 0)))
+    W = prepare(args.workload, args.scale, dev, args.n_train, keep_raw=args.search_dims > 0,
+                entry_links=links)
     nq = W["Q"].shape[0]
     k = 10
     ctx = Context(0)
@@ -502,9 +631,10 @@ def main():
         torch.cuda.synchronize()
         return D.recall(ids[:, :k].cpu().numpy(), W["gt"], 10)
 
-    beams = (args.beam,) if args.beam else (128, 256) if mixed else (32, 64, 128, 256)
+    beams = (args.beam,) if args.beam else (128, 256, 512) if mixed else (32, 64, 128, 256)
     iters = (args.max_iter or 100,) if args.beam else (10, 15, 20, 25, 30, 35, 40, 45, 50, 55, 60,
-                                                       70, 80, 100, 150, 200)
+                                                       70, 80, 100, 150, 200) + (
+        (250, 300) if mixed else ())
 
     def setting(spec, name):
         """Fixed (beam, max_iter) from 'B,I', or the sweep."""

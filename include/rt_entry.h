@@ -218,6 +218,19 @@ int rte_knn_rows(rte_ctx* ctx, const float* X, int64_t np, int d, int64_t stride
 int rte_graph_from_knn(rte_ctx* ctx, const int32_t* knn, int64_t np, int kf, int degree,
                        int rev_rank, int32_t* graph_out, int where);
 
+/* Entry-space links (no reference counterpart; a graph-construction option):
+ * the last r slots of every row of graph (np x degree, padded with np, host
+ * or device, changed in place) become points drawn from the kd-tree leaves
+ * l-window .. l+window around the row's own leaf l (leaf ids are in kd-tree
+ * DFS order: a neighbourhood in the 3-D entry space), excluding the node
+ * itself, its kept entries and each other (padding np after 8 draws).  On
+ * data of many well-separated clusters the kNN graph has no edges between
+ * clusters; these let a search whose entry seeds miss the query's cluster
+ * reach the points that share the entry region with it.  Deterministic in
+ * seed; needs rte_set_leaves (its pid table).  0 <= r <= min(degree, 32). */
+int rte_graph_add_entry_links(rte_ctx* ctx, int32_t* graph, int64_t np, int degree, int r,
+                              int window, uint64_t seed, int where);
+
 /* Batched uneven search: per-query k and beam, one persistent launch with a
  * device work queue (longest beams first), captured as a hipGraph on first
  * use and replayed while the launch shape is unchanged; no host sync.  Per

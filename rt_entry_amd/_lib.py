@@ -26,7 +26,7 @@ EXPORTS = [
     "rte_set_leaves", "rte_build_kdtree", "rte_expand_aabbs", "rte_select_entry",
     "rte_search", "rte_search_batched", "rte_search_batched_noalloc", "rte_autotune_batched",
     "rte_query", "rte_mlp_create", "rte_mlp_train_step",
-    "rte_mlp_predict", "rte_predict_entry", "rte_set_rerank_base", "rte_rerank", "rte_knn", "rte_knn_rows", "rte_graph_from_knn", "rte_pca_fit", "rte_pca_rotate", "rte_build_kdtree_gpu", "rte_mlp_get", "rte_mlp_set", "rte_mlp_grad", "rte_mlp_set_input_norm", "rte_hash_parameter",
+    "rte_mlp_predict", "rte_predict_entry", "rte_set_rerank_base", "rte_rerank", "rte_knn", "rte_knn_rows", "rte_graph_from_knn", "rte_graph_add_entry_links", "rte_pca_fit", "rte_pca_rotate", "rte_build_kdtree_gpu", "rte_mlp_get", "rte_mlp_set", "rte_mlp_grad", "rte_mlp_set_input_norm", "rte_hash_parameter",
     "rte_search_supported", "rte_version",
 ]
 
@@ -90,6 +90,7 @@ def lib():
         "rte_knn": ([P, P, i64, i32, i64, i32, P, P, i32], i32),
         "rte_knn_rows": ([P, P, i64, i32, i64, i64, i64, i32, P, P, i32], i32),
         "rte_graph_from_knn": ([P, P, i64, i32, i32, i32, P, i32], i32),
+        "rte_graph_add_entry_links": ([P, P, i64, i32, i32, i32, C.c_uint64, i32], i32),
         "rte_pca_fit": ([P, P, i64, i32, i64, P, P, P, i32], i32),
         "rte_pca_rotate": ([P, P, i64, i32, i64, P, P, i32, P, i32], i32),
         "rte_build_kdtree_gpu": ([P, P, i64, i64, i32, i32, P, P, P, i32], i32),

@@ -434,6 +434,20 @@ class Context:
                                            int(rev_rank), pg, w))
         return g
 
+    def graph_add_entry_links(self, graph, r: int, window: int = 0, seed: int = 1):
+        """rte_graph_add_entry_links on a graph (host array: a modified copy
+        is returned; device tensor: changed in place and returned).  Needs
+        set_leaves first."""
+        if _is_torch(graph) and graph.is_cuda:
+            g = graph
+        else:
+            g = np.array(graph, np.int32, copy=True)
+        pg, w, keep = _ptr(g, np.int32)
+        n, deg = g.shape
+        self._chk(lib().rte_graph_add_entry_links(self._h, pg, int(n), int(deg), int(r),
+                                                  int(window), C.c_uint64(seed), w))
+        return g
+
     def set_rerank_base(self, base):
         """Full-precision vectors for rte_rerank (np x d f32, host or device)."""
         p, w, keep = _ptr(base, np.float32)

@@ -19,6 +19,7 @@
 #include "project.h"
 #include "rerank.h"
 #include "knn.h"
+#include "links.h"
 #include "search.h"
 
 namespace {
@@ -361,6 +362,36 @@ int rte_graph_from_knn(rte_ctx* c, const int32_t* knn, int64_t np, int kf, int d
          "graph launch");
   if (int r = finish_out(c, graph_out, dg, (size_t)np * degree, where)) return r;
   HIPCHK(c, hipStreamSynchronize(c->stream), "sync graph");
+  return RTE_OK;
+}
+
+int rte_graph_add_entry_links(rte_ctx* c, int32_t* graph, int64_t np, int degree, int r,
+                              int window, uint64_t seed, int where) {
+  if (!c) return RTE_EINVAL;
+  if (!c->pids || c->n_leaf <= 0) return fail(c, RTE_ESTATE, "rte_set_leaves not called");
+  if (!graph || np <= 0 || np >= 0x7fffffff || degree < 1 || degree > 1024)
+    return fail(c, RTE_EINVAL, "bad graph shape np=%lld degree=%d", (long long)np, degree);
+  if (r < 0 || r > degree || r > rte::kMaxLinks || window < 0)
+    return fail(c, RTE_EINVAL, "links need 0 <= r <= min(degree, %d) and window >= 0 (r %d)",
+                rte::kMaxLinks, r);
+  if (r == 0) return RTE_OK;
+  if (int e = set_device(c)) return e;
+  DevBuf gbuf, leaf_of, size;
+  int32_t* dg = graph;
+  const size_t bytes = (size_t)np * degree * sizeof(int32_t);
+  if (where == RTE_HOST) {
+    HIPCHK(c, gbuf.reserve(bytes), "hipMalloc(graph)");
+    HIPCHK(c, hipMemcpyAsync(gbuf.p, graph, bytes, hipMemcpyHostToDevice, c->stream), "copy graph");
+    dg = gbuf.as<int32_t>();
+  }
+  HIPCHK(c, leaf_of.reserve((size_t)np * sizeof(int32_t)), "hipMalloc(leaf index)");
+  HIPCHK(c, size.reserve((size_t)c->n_leaf * sizeof(int32_t)), "hipMalloc(leaf sizes)");
+  HIPCHK(c, rte::launch_entry_links(dg, np, degree, r, window, seed, c->pids, c->n_leaf, c->cap,
+                                    leaf_of.as<int32_t>(), size.as<int32_t>(), c->stream),
+         "entry links");
+  if (where == RTE_HOST)
+    HIPCHK(c, hipMemcpyAsync(graph, dg, bytes, hipMemcpyDeviceToHost, c->stream), "copy graph");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "sync links");
   return RTE_OK;
 }
 
@@ -887,9 +918,8 @@ static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
       int wmax = 0;
       for (int lg = 5; lg <= 9; ++lg) {
         int w, h;
-        bloom_for(p, 1 << lg, &w, &h);
-        a.auto_words[lg] = w;
-        a.auto_hashes[lg] = h;
+        bloom_for(p, 1 << lg, &w, &h);  // a power of two below 2^15, 7 hashes
+        a.auto_lgw |= (uint64_t)ilog2(w) << (4 * lg);
         wmax = std::max(wmax, w);
       }
       a.bloom_words_max = wmax;

@@ -35,9 +35,11 @@ struct SearchArgs {
   int bloom_words;
   int bloom_hashes;
   int bloom_words_max;  // LDS words reserved per wave for the filter
-  int bloom_auto;       // per-query sizing from auto_words/auto_hashes
-  int auto_words[16];   // indexed by log2(beam)
-  int auto_hashes[16];
+  int bloom_auto;       // per-query filter sizing from auto_lgw
+  // log2 of the filter words for beam 2^i in bits [4i, 4i + 4) (filter
+  // sizes are powers of two; one 64-bit field instead of a table, which the
+  // kernel would hold in scalar registers for its whole life)
+  uint64_t auto_lgw;
   const int32_t* k_per_q;     // uneven batches (or null)
   const int32_t* beam_per_q;  // uneven batches (or null)
   int64_t* work_counter;      // persistent launch: shared query counter (or null)

@@ -657,27 +657,19 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
     // them are scalar branches, not exec-masked ones
     n = __builtin_amdgcn_readfirstlane(n);
     k = __builtin_amdgcn_readfirstlane(k);
-    int bwords = a.bloom_words, bhash = a.bloom_hashes;
+    int bwords = a.bloom_words;
     if (a.bloom_auto) {  // auto_tune_bloom sizing per beam
-      // constant indices only: a computed index into the by-value argument
-      // block would turn these reads into generic (flat) loads
       const int lg = 31 - __clz(n);
-#pragma unroll
-      for (int t = 0; t < 16; ++t)
-        if (t == lg) {
-          bwords = a.auto_words[t];
-          bhash = a.auto_hashes[t];
-        }
+      bwords = 1 << (int)((a.auto_lgw >> (4 * lg)) & 15u);
     }
     bwords = __builtin_amdgcn_readfirstlane(bwords);
-    bhash = __builtin_amdgcn_readfirstlane(bhash);
     WaveLds s;
     s.bloom = base_lds;
     s.cid = (int*)(base_lds + a.bloom_words_max);
     s.sbd = (float*)(s.cid + 64 * MR);
     s.sbi = (int*)(s.sbd + 64 * MR);
     s.cdist = WPQ > 1 ? nullptr : (float*)(s.sbi + 64 * MR);
-    (void)bhash;  // fixed per mode (kHashes), checked on the host
+    // the hash count is fixed per mode (kHashes), checked on the host
     Bloom<kHashes<TUNED>> bl{s.bloom, (uint32_t)bwords, (uint32_t)(31 - __clz(bwords)),
                              (uint32_t)bwords * 32u};
 

@@ -281,6 +281,62 @@ def graph_from_knn(knn: np.ndarray, degree: int, rev_rank: int | None = None) ->
     return g
 
 
+def _splitmix64(z: np.ndarray) -> np.ndarray:
+    z = z.astype(np.uint64)
+    z ^= z >> np.uint64(30)
+    z *= np.uint64(0xBF58476D1CE4E5B9)
+    z ^= z >> np.uint64(27)
+    z *= np.uint64(0x94D049BB133111EB)
+    z ^= z >> np.uint64(31)
+    return z
+
+
+def add_entry_links(g: np.ndarray, pids: np.ndarray, r: int, window: int = 0, seed: int = 1,
+                    tries: int = 8) -> np.ndarray:
+    """Numpy definition of rte_graph_add_entry_links: the last r slots of
+    row u become points of the leaves within `window` ids of u's leaf (in
+    the pid table's order), candidate t of slot j drawn as
+    splitmix64(seed + ((u << 16) | (j << 8) | t) * 0x9E3779B97F4A7C15) mod
+    the window's point count, redrawn when equal to u, a kept entry or an
+    earlier link, padding (n) after `tries` draws."""
+    n, degree = g.shape
+    n_leaf, cap = pids.shape
+    out = np.array(g, np.int32, copy=True)
+    if r == 0:
+        return out
+    leaf_of = np.full(n, -1, np.int64)
+    valid = pids < n
+    leaf_of[pids[valid]] = np.repeat(np.arange(n_leaf), cap).reshape(n_leaf, cap)[valid]
+    size = valid.sum(1).astype(np.uint64)
+    keep = degree - r
+    golden = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        for u in range(n):
+            l = leaf_of[u]
+            if l < 0:
+                out[u, keep:] = n
+                continue
+            lo, hi = max(0, l - window), min(n_leaf - 1, l + window)
+            total = size[lo:hi + 1].sum()
+            links = []
+            for j in range(r):
+                pick = n
+                for t in range(tries):
+                    key = np.uint64((u << 16) | (j << 8) | t)
+                    idx = int(_splitmix64(np.array([np.uint64(seed) + key * golden]))[0] % total)
+                    x = lo
+                    while idx >= int(size[x]):
+                        idx -= int(size[x])
+                        x += 1
+                    c = int(pids[x, idx])
+                    if c != u and c not in out[u, :keep] and c not in links:
+                        pick = c
+                        break
+                links.append(pick)
+            out[u, keep:] = links
+    return out
+
+
 def pad_graph(g: np.ndarray, n: int) -> np.ndarray:
     """Pad rows to a power-of-two stride with n (graph.cu:134)."""
     deg = g.shape[1]

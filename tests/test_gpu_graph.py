@@ -147,3 +147,32 @@ def test_pca_rotate_bit_exact(ctx, n, d, d_out, device):
     if device:
         out = out.cpu().numpy()
     np.testing.assert_array_equal(out, _rotate_ordered(X, R[:, :d_out], mu))
+
+
+def test_entry_links_equal_reference(ctx, int_workload):
+    """rte_graph_add_entry_links equals data.add_entry_links bit for bit
+    (host and device graphs, several r and windows), and a search on the
+    linked graph equals the oracle's on the same graph."""
+    import torch
+    import oracle as O
+    from conftest import assert_same
+    from rt_entry_amd import data as D
+    from rt_entry_amd import make_params
+    w = int_workload
+    ctx.set_base(w.base)
+    ctx.set_leaves(w.aabbs, w.pids)
+    for r, win, seed in ((4, 0, 1), (8, 2, 7), (32, 1, 3)):
+        print(f"[links] r={r} window={win} seed={seed}", flush=True)
+        ref = D.add_entry_links(w.graph_raw, w.pids, r, window=win, seed=seed)
+        gh = ctx.graph_add_entry_links(w.graph_raw, r, window=win, seed=seed)
+        assert_same(gh, ref, f"host r={r}")
+        gd = torch.from_numpy(w.graph_raw.copy()).cuda()
+        ctx.graph_add_entry_links(gd, r, window=win, seed=seed)
+        assert_same(gd.cpu().numpy(), ref, f"device r={r}")
+    g = D.add_entry_links(w.graph_raw, w.pids, 8, window=1, seed=5)
+    ctx.set_graph(g)
+    io, do = O.search(w.base, D.pad_graph(g, w.n), w.queries, w.pids, w.hits,
+                      O.make_params(k=10, beam=64))
+    ig, dg = ctx.search(w.queries, w.hits, make_params(k=10, beam=64))
+    assert_same(ig, io, "ids on the linked graph")
+    assert_same(dg.view(np.int32), do.view(np.int32), "dists on the linked graph")

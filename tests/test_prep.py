@@ -159,3 +159,35 @@ def test_recall_matches_oracle():
     ids = np.zeros((1, 10), np.int32)
     ids[0, 1] = 1
     assert D.recall(ids, np.array([[1, 2, 3, 4, 5, 6, 7, 8, 9, 10]], np.int32), 10) == 0.1
+
+
+def test_entry_links_reference_rules():
+    """data.add_entry_links (the definition rte_graph_add_entry_links is
+    checked against): kept entries unchanged, links drawn from the leaves
+    within `window` of the node's own leaf, never the node itself, a kept
+    entry or a repeat; deterministic in the seed; r = 0 is the identity."""
+    import oracle as O
+    from rt_entry_amd import data as D
+    rng = np.random.default_rng(11)
+    n = 2500
+    X = rng.normal(size=(n, 3)).astype(np.float32)
+    boxes, belong = O.kdtree_build(X, 64)
+    pids = O.aabb_pids(belong, boxes.shape[0], 64)
+    g = np.full((n, 16), n, np.int32)
+    for u in range(n):
+        c = rng.choice(n, 13, replace=False)
+        c = c[c != u][:12]
+        g[u, :c.size] = c
+    assert_same(D.add_entry_links(g, pids, 0), g, "r = 0")
+    for r, w in ((4, 0), (6, 2)):
+        a = D.add_entry_links(g, pids, r, window=w, seed=9)
+        assert_same(a, D.add_entry_links(g, pids, r, window=w, seed=9), "deterministic")
+        assert not np.array_equal(a, D.add_entry_links(g, pids, r, window=w, seed=10))
+        keep = 16 - r
+        assert_same(a[:, :keep], g[:, :keep], "kept entries")
+        for u in range(n):
+            links = [v for v in a[u, keep:].tolist() if v != n]
+            assert len(links) == len(set(links)) and u not in links
+            assert not set(links) & set(a[u, :keep].tolist())
+            assert all(abs(int(belong[v]) - int(belong[u])) <= w for v in links)
+        assert (a[:, keep:] != n).mean() > 0.99
