@@ -36,8 +36,8 @@ WORKLOAD_NAMES = {1: "Gaussian mixture (64 centres)", 2: "integer-valued mixture
                   5: "integer-valued mixture (10k centres)"}
 
 
-SEARCH_SOURCES = ("search_impl.h", "search.h", "search.hip", "search_nr1.hip", "search_nr2.hip",
-                  "search_nr4.hip", "search_nr8.hip")
+SEARCH_SOURCES = ("search_impl.h", "search.h", "search.hip") + tuple(
+    f"search_nr{nr}{v}.hip" for nr in (1, 2, 4, 8) for v in ("", "_mw", "_lq"))
 
 
 def kernel_source_id() -> str:
@@ -64,7 +64,8 @@ GRID_CODES = {0: "resident", -1: "2x resident", -2: "balanced rounds", -3: "one 
 def launch_desc(p) -> str:
     """The batched launch shape of a tuned parameter set."""
     grid = GRID_CODES.get(p.grid, str(p.grid))
-    return (f"batched wpq={max(1, p.wpq)} wg_waves={p.wg_waves} grid={grid} "
+    return (f"batched wpq={max(1, p.wpq)} wg_waves={p.wg_waves} "
+            f"queue={'lds' if p.queue == 1 else 'vgpr'} grid={grid} "
             f"order={'leaf' if p.order == 1 else 'given'} bloom_words={p.bloom_words or 'default'}")
 
 
@@ -365,15 +366,16 @@ def bench_uneven(args):
     print(json.dumps(out))
 
 
-LAUNCH_VARIANTS = (  # (wpq, wg_waves, order, grid code)
-    (1, 4, 0, 0), (1, 4, 1, 0), (1, 2, 0, 0), (1, 2, 1, 0), (1, 1, 1, 0), (1, 4, 0, -2),
-    (2, 2, 0, 0), (2, 2, 1, 0), (2, 2, 0, -3), (4, 4, 0, 0), (4, 4, 0, -3))
+LAUNCH_VARIANTS = (  # (wpq, wg_waves, order, grid code, queue)
+    (1, 4, 0, 0, 0), (1, 4, 1, 0, 0), (1, 2, 0, 0, 0), (1, 2, 1, 0, 0), (1, 1, 1, 0, 0),
+    (1, 4, 0, -2, 0), (1, 4, 1, 0, 1), (1, 2, 1, 0, 1), (1, 1, 1, 0, 1), (1, 2, 0, -3, 1),
+    (2, 2, 0, 0, 0), (2, 2, 1, 0, 0), (2, 2, 0, -3, 0), (4, 4, 0, 0, 0), (4, 4, 0, -3, 0))
 
 
 def launch_table(args):
     """Every launch variant of the search on one workload, kd-tree entries:
     the plain rte_search launch, the batched launch for each (waves per
-    query, queries per workgroup, query order, grid code), and the
+    query, queries per workgroup, query order, grid code, beam storage), and the
     autotuner's choice; each timed with HIP events on the library's stream
     over --steps launches after one untimed launch, ids checked equal to the
     plain launch's.  One JSON line per launch; --nq takes the first n
@@ -433,9 +435,9 @@ def launch_table(args):
                 stt = stt.cpu().numpy().astype(np.int64)
                 alg += float((4 * d * (stt[:, 0] + 1) + 4 * Dg * stt[:, 1] + 4 * bb
                               + 8 * kk).sum())
-        for wpq, wg, order, grid in LAUNCH_VARIANTS:
+        for wpq, wg, order, grid, queue in LAUNCH_VARIANTS:
             p = make_params(k=kmax, beam=256, max_iter=args.max_iter or 100, mode=args.mode,
-                            wpq=wpq, wg_waves=wg, order=order, grid=grid)
+                            wpq=wpq, wg_waves=wg, order=order, grid=grid, queue=queue)
             ms = timed(lambda: ctx.search_batched(pq, hits, kd_t, bd_t, p, k_stride=kmax,
                                                   out=(ids, dists)))
             emit(dict(workload=3, nq=nq, launch=launch_desc(p)), alg, ms)
@@ -450,9 +452,9 @@ def launch_table(args):
         emit(dict(workload=args.workload, nq=nq, beam=beam, max_iter=mi, launch="grid",
                   recall=round(D.recall(ids.cpu().numpy(), W["gt"][:nq], 10), 4)), alg, ms)
         i2, d2 = torch.empty_like(ids), torch.empty_like(dd)
-        for wpq, wg, order, grid in LAUNCH_VARIANTS:
+        for wpq, wg, order, grid, queue in LAUNCH_VARIANTS:
             pb = make_params(k=10, beam=beam, max_iter=mi, mode=args.mode, wpq=wpq, wg_waves=wg,
-                             order=order, grid=grid)
+                             order=order, grid=grid, queue=queue)
             ms = timed(lambda: ctx.search_batched(pq, hits, None, None, pb, k_stride=10,
                                                   out=(i2, d2)))
             emit(dict(workload=args.workload, nq=nq, beam=beam, max_iter=mi,
@@ -460,6 +462,13 @@ def launch_table(args):
         best, tms = ctx.autotune_batched(pq, hits, None, None, p, k_stride=10)
         print(json.dumps(dict(workload=args.workload, nq=nq, beam=beam, max_iter=mi,
                               autotuned=launch_desc(best), ms=round(tms, 4))), flush=True)
+    if args.copy_mib:  # known-size copies that scale the byte counters
+        a = torch.empty(args.copy_mib << 18, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
 
 
 def main():
@@ -594,7 +603,8 @@ def main():
     def with_k(p, kk):
         return make_params(k=kk, beam=p.beam, width=p.width, max_iter=p.max_iter, mode=p.mode,
                            algo=p.algo, bloom_words=p.bloom_words, bloom_hashes=p.bloom_hashes,
-                           wg_waves=p.wg_waves, grid=p.grid, wpq=p.wpq, order=p.order)
+                           wg_waves=p.wg_waves, grid=p.grid, wpq=p.wpq, order=p.order,
+                           queue=p.queue)
 
     def cand_bufs(beam):
         """Whole-beam outputs of the truncated search (rerank candidates)."""

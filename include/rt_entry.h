@@ -71,6 +71,11 @@ typedef struct rte_params {
                        device, the sorted range cut into one piece per XCD
                        (workgroups blockIdx % 8), so queries that gather
                        the same rows share an L2; never changes results */
+  int queue;        /* where the beam (candidate queue) lives: RTE_QUEUE_VGPR
+                       (0, registers) or RTE_QUEUE_LDS (an LDS queue of
+                       64 * ceil(beam bound / 64) slots per wave, 8 bytes a
+                       slot; fewer VGPRs for large beams, wpq 0 or 1 only);
+                       never changes results */
 } rte_params;
 
 /* rte_params.grid codes, resolved per call from the launched kernel's
@@ -85,6 +90,9 @@ typedef struct rte_params {
 
 #define RTE_ORDER_GIVEN 0
 #define RTE_ORDER_LEAF 1
+
+#define RTE_QUEUE_VGPR 0
+#define RTE_QUEUE_LDS 1
 
 /* ---- context ---------------------------------------------------------- */
 /* Replaces Graph::Graph + cublasCreate (graph.cu:32-64). */
@@ -273,13 +281,14 @@ int rte_search_batched_noalloc(rte_ctx* ctx, const float* PQ, int64_t nq, int64_
 
 /* Autotuner for rte_search_batched: times the batch (device buffers) under
  * every launch shape -- wavefronts per query (wpq 1, 2, 4) x queries per
- * workgroup (wg_waves 1, 2, 4 at wpq 1) x grid (RTE_GRID_PER_QUERY,
- * RTE_GRID_DEFAULT, RTE_GRID_RESIDENT2, RTE_GRID_BALANCED, each resolved
- * from the occupancy of the kernel launched) x query order (RTE_ORDER_GIVEN,
- * RTE_ORDER_LEAF with leaf entries) -- and every filter size in
- * bloom_words[0..n_bloom) (0 = the mode's default), and writes the fastest
- * configuration to *best (a copy of *p with wpq, wg_waves, grid, order and
- * bloom_words set) and its mean time to *best_ms.  A shape that fails is
+ * workgroup (wg_waves 1, 2, 4 at wpq 1) x beam storage (RTE_QUEUE_VGPR, and
+ * RTE_QUEUE_LDS at wpq 1) x grid (RTE_GRID_PER_QUERY, RTE_GRID_DEFAULT,
+ * RTE_GRID_RESIDENT2, RTE_GRID_BALANCED, each resolved from the occupancy
+ * of the kernel launched) x query order (RTE_ORDER_GIVEN, RTE_ORDER_LEAF
+ * with leaf entries) -- and every filter size in bloom_words[0..n_bloom)
+ * (0 = the mode's default), and writes the fastest configuration to *best
+ * (a copy of *p with wpq, wg_waves, queue, grid, order and bloom_words set)
+ * and its mean time to *best_ms.  A shape that fails is
  * skipped; the call fails only when every shape did (with the first
  * failure).  Launch shape and order never change results; the filter size
  * can, so callers pass only sizes they accept.  ids_out/dists_out receive

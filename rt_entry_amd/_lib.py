@@ -40,7 +40,7 @@ class RteError(RuntimeError):
 class Params(C.Structure):
     _fields_ = [(n, C.c_int) for n in (
         "k", "beam", "width", "max_iter", "mode", "algo", "bloom_words", "bloom_hashes",
-        "wg_waves", "grid", "wpq", "order")]
+        "wg_waves", "grid", "wpq", "order", "queue")]
 
 
 _lib = None

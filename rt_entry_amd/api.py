@@ -41,10 +41,10 @@ def _ptr(x, dtype):
 
 
 def make_params(k=10, beam=64, width=1, max_iter=100, mode=MODE_REF, algo=1, bloom_words=0,
-                bloom_hashes=0, wg_waves=0, grid=0, wpq=0, order=0) -> Params:
+                bloom_hashes=0, wg_waves=0, grid=0, wpq=0, order=0, queue=0) -> Params:
     return Params(int(k), int(beam), int(width), int(max_iter), int(mode), int(algo),
                   int(bloom_words), int(bloom_hashes), int(wg_waves), int(grid), int(wpq),
-                  int(order))
+                  int(order), int(queue))
 
 
 def hash_parameter(nc: int):

@@ -27,7 +27,8 @@ HOST = os.path.join(HERE, "host")
 
 HIP_SOURCES = ["api.hip", "search.hip", "search_nr1.hip", "search_nr2.hip", "search_nr4.hip",
                "search_nr8.hip", "search_nr1_mw.hip", "search_nr2_mw.hip", "search_nr4_mw.hip",
-               "search_nr8_mw.hip", "noalloc.hip", "links.hip", "entry.hip", "project.hip", "mlp.hip", "rerank.hip", "knn.hip", "pca.hip", "kdtree_gpu.hip"]
+               "search_nr8_mw.hip", "search_nr1_lq.hip", "search_nr2_lq.hip",
+               "search_nr4_lq.hip", "search_nr8_lq.hip", "noalloc.hip", "links.hip", "entry.hip", "project.hip", "mlp.hip", "rerank.hip", "knn.hip", "pca.hip", "kdtree_gpu.hip"]
 CXX_SOURCES = ["kdtree.cpp"]
 
 HIPCC_FLAGS = [

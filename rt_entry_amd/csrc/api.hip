@@ -960,6 +960,11 @@ static int search_common(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
   if (p->grid < RTE_GRID_PER_QUERY) return fail(c, RTE_EINVAL, "unknown grid code %d", p->grid);
   if (p->order != RTE_ORDER_GIVEN && p->order != RTE_ORDER_LEAF)
     return fail(c, RTE_EINVAL, "unknown order %d", p->order);
+  if (p->queue != RTE_QUEUE_VGPR && p->queue != RTE_QUEUE_LDS)
+    return fail(c, RTE_EINVAL, "unknown queue %d", p->queue);
+  if (p->queue == RTE_QUEUE_LDS && a.wpq > 1)
+    return fail(c, RTE_EINVAL, "the LDS queue runs one wave per query (wpq %d)", p->wpq);
+  a.queue = p->queue;
   a.queries = dq;
   a.hits = dleaf;
   a.ids_out = dids;
@@ -1103,27 +1108,29 @@ int rte_autotune_batched(rte_ctx* c, const float* PQ, int64_t nq, int64_t q_stri
   if (!p || !best || !best_ms || !bloom_words || n_bloom < 1)
     return fail(c, RTE_EINVAL, "bad autotune arguments");
   if (int r = set_device(c)) return r;
-  // candidates: waves per query x queries per workgroup x grid code x query
-  // order x filter size; every grid code is resolved by the call itself from
-  // the occupancy of the kernel it launches
+  // candidates: waves per query x queries per workgroup x beam storage x
+  // grid code x query order x filter size; every grid code is resolved by
+  // the call itself from the occupancy of the kernel it launches
   std::vector<rte_params> cands;
   const int orders[2] = {RTE_ORDER_GIVEN, RTE_ORDER_LEAF};
   for (int bi = 0; bi < n_bloom; ++bi)
     for (int wpq : {1, 2, 4})
-      for (int wpb : {1, 2, 4}) {
-        if (wpq > 1 && wpb != wpq) continue;
-        for (int g : {RTE_GRID_PER_QUERY, RTE_GRID_DEFAULT, RTE_GRID_RESIDENT2,
-                      RTE_GRID_BALANCED})
-          for (int o = 0; o < (p->algo == 1 ? 2 : 1); ++o) {
-            rte_params t = *p;
-            t.bloom_words = bloom_words[bi];
-            t.wpq = wpq;
-            t.wg_waves = wpb;
-            t.grid = g;
-            t.order = orders[o];
-            cands.push_back(t);
-          }
-      }
+      for (int wpb : {1, 2, 4})
+        for (int qu : {RTE_QUEUE_VGPR, RTE_QUEUE_LDS}) {
+          if (wpq > 1 && (wpb != wpq || qu == RTE_QUEUE_LDS)) continue;
+          for (int g : {RTE_GRID_PER_QUERY, RTE_GRID_DEFAULT, RTE_GRID_RESIDENT2,
+                        RTE_GRID_BALANCED})
+            for (int o = 0; o < (p->algo == 1 ? 2 : 1); ++o) {
+              rte_params t = *p;
+              t.bloom_words = bloom_words[bi];
+              t.wpq = wpq;
+              t.wg_waves = wpb;
+              t.queue = qu;
+              t.grid = g;
+              t.order = orders[o];
+              cands.push_back(t);
+            }
+        }
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0), "event");
   HIPCHK(c, hipEventCreate(&e1), "event");

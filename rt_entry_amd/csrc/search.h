@@ -48,6 +48,9 @@ struct SearchArgs {
                               // counters (work_counter + 16 * range), 0/1: one
   int wpq;                    // waves per query (1, 2, 4); > 1: one query per
                               // workgroup of wpq waves
+  int queue;                  // where the beam lives: 0 VGPRs, 1 an LDS queue
+                              // of 64 * ceil(beam bound / 64) slots per wave
+                              // (one wave per query)
 };
 
 int search_supported(int beam_max, int m, int d);
@@ -57,10 +60,10 @@ hipError_t launch_search(const SearchArgs& a, int beam_max, int grid, int wpb, h
 // resident workgroups per CU for this launch shape
 hipError_t search_occupancy(const SearchArgs& a, int beam_max, int wpb, int* blocks_per_cu);
 
-// launchers per beam register count, batch register count and waves per
-// query (translation units search_nr*.hip); with occ non-null they report
-// occupancy instead of launching
-template <int NR, int MR, int WPQ>
+// launchers per beam register count, batch register count, waves per query
+// and beam storage (translation units search_nr*.hip); with occ non-null
+// they report occupancy instead of launching
+template <int NR, int MR, int WPQ, bool LQ>
 hipError_t launch_nmw(const SearchArgs& a, int grid, int wpb, hipStream_t st, int* occ);
 
 }  // namespace rte

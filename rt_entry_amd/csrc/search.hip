@@ -20,11 +20,12 @@ static int nr_of(int beam_max) {
 
 template <int NR, int MR>
 static hipError_t by_wpq(const SearchArgs& a, int grid, int wpb, hipStream_t st, int* occ) {
+  if (a.queue == 1) return launch_nmw<NR, MR, 1, true>(a, grid, wpb, st, occ);
   switch (a.wpq) {
     case 0:
-    case 1: return launch_nmw<NR, MR, 1>(a, grid, wpb, st, occ);
-    case 2: return launch_nmw<NR, MR, 2>(a, grid, wpb, st, occ);
-    case 4: return launch_nmw<NR, MR, 4>(a, grid, wpb, st, occ);
+    case 1: return launch_nmw<NR, MR, 1, false>(a, grid, wpb, st, occ);
+    case 2: return launch_nmw<NR, MR, 2, false>(a, grid, wpb, st, occ);
+    case 4: return launch_nmw<NR, MR, 4, false>(a, grid, wpb, st, occ);
   }
   return hipErrorInvalidValue;
 }
@@ -47,6 +48,8 @@ static hipError_t dispatch(const SearchArgs& a, int beam_max, int grid, int wpb,
 }
 
 static bool shape_ok(const SearchArgs& a, int wpb) {
+  if (a.queue != 0 && a.queue != 1) return false;
+  if (a.queue == 1 && a.wpq > 1) return false;  // the LDS queue: one wave per query
   if (a.wpq > 1) return (a.wpq == 2 || a.wpq == 4) && wpb == a.wpq;
   return a.wpq >= 0 && wpb >= 1 && wpb <= 4;
 }

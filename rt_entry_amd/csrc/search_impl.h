@@ -4,7 +4,8 @@
 //  - one wavefront owns one query; a 256-thread workgroup carries 4
 //    independent queries and never uses a workgroup barrier;
 //  - the beam L (N slots) and the expansion batch B (M = width * 2^shift
-//    slots) live in VGPRs, slot s in lane s % 64 of register s / 64; the
+//    slots) live in VGPRs, slot s in lane s % 64 of register s / 64 (or,
+//    for L, an LDS queue that the merge loads into that layout); the
 //    reference's compare-exchange networks (bitonic sort of B
 //    :888-919/:1424-1453, min-exchange :921-931/:1455-1467, half-cleaner
 //    merge of L :933-958/:1469-1493) run as cross-lane exchanges (DPP and
@@ -186,9 +187,9 @@ __device__ __forceinline__ void key_masks(float md, int mi, float pd, int pi, ui
 // larger, the other lane only when it is strictly smaller -- exactly the
 // reference's swap on a strict compare.  The direction pattern is a
 // compile-time lane mask, so the choice is scalar-unit work.
-template <bool TUNED, int SUB, class Dir, int R, int r>
+template <bool TUNED, int SUB, class Dir, int R, int RA, int r>
 __device__ __forceinline__ void stage_reg(float (&d)[R], int (&id)[R], int lane) {
-  if constexpr (r < R) {
+  if constexpr (r < RA) {
     constexpr uint64_t W = want_larger_mask<SUB, Dir>(r);
     const float pd = xor_lane<SUB>(d[r], lane);
     const int pi = xor_lane<SUB>(id[r], lane);
@@ -197,18 +198,20 @@ __device__ __forceinline__ void stage_reg(float (&d)[R], int (&id)[R], int lane)
     const bool take = __builtin_amdgcn_inverse_ballot_w64((pgt & W) | (plt & ~W));
     d[r] = take ? pd : d[r];
     id[r] = take ? pi : id[r];
-    stage_reg<TUNED, SUB, Dir, R, r + 1>(d, id, lane);
+    stage_reg<TUNED, SUB, Dir, R, RA, r + 1>(d, id, lane);
   }
 }
 
-// Network stage over an R-register array (element e = r*64 + lane) at a
-// compile-time distance SUB.
-template <bool TUNED, int R, int SUB, class Dir>
+// Network stage over the first RA registers of an R-register array (element
+// e = r*64 + lane) at a compile-time distance SUB.  A network over the first
+// N elements pairs them only among themselves, so registers past
+// ceil(N / 64) (padding) are left out.
+template <bool TUNED, int R, int SUB, class Dir, int RA = R>
 __device__ __forceinline__ void net_stage(float (&d)[R], int (&id)[R], int lane) {
   if constexpr (SUB >= 64) {
     constexpr int RS = SUB / 64;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
+    for (int r = 0; r < RA; ++r) {
       if ((r & RS) == 0) {
         // the direction depends on r only (2*STEP >= 128 here)
         const bool dsc = Dir::desc(r * 64);
@@ -222,22 +225,27 @@ __device__ __forceinline__ void net_stage(float (&d)[R], int (&id)[R], int lane)
       }
     }
   } else {
-    stage_reg<TUNED, SUB, Dir, R, 0>(d, id, lane);
+    stage_reg<TUNED, SUB, Dir, R, RA, 0>(d, id, lane);
   }
 }
 
 // Bitonic sort, descending overall, of the first M_ elements (M_ a power of
 // two, M_ <= 64*R): graph_search.cuh:888-919.  Sizes are compile-time so a
 // network is straight-line code with no per-substep guards.
-template <bool TUNED, int R, int STEP, int SUB>
+// registers holding the first N elements
+template <int N, int R>
+constexpr int regs_of() {
+  return (N + 63) / 64 < 1 ? 1 : ((N + 63) / 64 > R ? R : (N + 63) / 64);
+}
+template <bool TUNED, int R, int STEP, int SUB, int RA>
 __device__ __forceinline__ void sort_sub(float (&d)[R], int (&id)[R], int lane) {
-  net_stage<TUNED, R, SUB, SortDir<STEP>>(d, id, lane);
-  if constexpr (SUB > 1) sort_sub<TUNED, R, STEP, SUB / 2>(d, id, lane);
+  net_stage<TUNED, R, SUB, SortDir<STEP>, RA>(d, id, lane);
+  if constexpr (SUB > 1) sort_sub<TUNED, R, STEP, SUB / 2, RA>(d, id, lane);
 }
 template <bool TUNED, int R, int M_, int STEP = 1>
 __device__ __forceinline__ void sort_net(float (&d)[R], int (&id)[R], int lane) {
   if constexpr (STEP <= M_ / 2) {
-    sort_sub<TUNED, R, STEP, STEP>(d, id, lane);
+    sort_sub<TUNED, R, STEP, STEP, regs_of<M_, R>()>(d, id, lane);
     sort_net<TUNED, R, M_, STEP * 2>(d, id, lane);
   }
 }
@@ -252,15 +260,15 @@ __device__ __forceinline__ void sort_desc(float (&d)[R], int (&id)[R], int m, in
 
 // Half-cleaner cascade, every pair ascending, substeps N_/2..1:
 // graph_search.cuh:933-958.
-template <bool TUNED, int R, int SUB>
+template <bool TUNED, int R, int SUB, int RA>
 __device__ __forceinline__ void merge_net(float (&d)[R], int (&id)[R], int lane) {
-  net_stage<TUNED, R, SUB, MergeDir>(d, id, lane);
-  if constexpr (SUB > 1) merge_net<TUNED, R, SUB / 2>(d, id, lane);
+  net_stage<TUNED, R, SUB, MergeDir, RA>(d, id, lane);
+  if constexpr (SUB > 1) merge_net<TUNED, R, SUB / 2, RA>(d, id, lane);
 }
 template <bool TUNED, int R, int N_ = 64 * R>
 __device__ __forceinline__ void merge_asc(float (&d)[R], int (&id)[R], int n, int lane) {
   if constexpr (N_ >= 2) {
-    if (n == N_) merge_net<TUNED, R, N_ / 2>(d, id, lane);
+    if (n == N_) merge_net<TUNED, R, N_ / 2, regs_of<N_, R>()>(d, id, lane);
     else merge_asc<TUNED, R, N_ / 2>(d, id, n, lane);
   }
 }
@@ -272,6 +280,137 @@ struct WaveLds {
   float* cdist;     // 64*MR compacted distances
   float* sbd;       // 64*MR sorted batch distances
   int* sbi;         // 64*MR sorted batch ids
+  float* qd;        // 64*NR beam distances (beam held in LDS)
+  int* qi;          // 64*NR beam ids (beam held in LDS)
+};
+
+// Expansion-flag ballots of the beam (graph_search.cuh:1014-1031): bf has
+// the slots whose id is in (0, np), i.e. not yet expanded (an expanded slot
+// holds -id), bz the slots holding id 0, whose sign cannot carry the flag.
+template <int NR>
+__device__ __forceinline__ void flag_ballots(const int (&li)[NR], int lane, int n, int64_t np,
+                                             uint64_t (&bf)[NR], uint64_t (&bz)[NR]) {
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (r * 64 >= n) {  // a register past the beam (uniform n): no slots
+      bf[r] = 0;
+      bz[r] = 0;
+      continue;
+    }
+    // slot r*64 + lane < n, compared as lane < n - r*64 so that no slot
+    // index per register stays live in a VGPR
+    const bool in = lane < n - r * 64;
+    bf[r] = __ballot(in && li[r] > 0 && (int64_t)li[r] < np);
+    bz[r] = __ballot(in && li[r] == 0);
+  }
+}
+
+// The beam L of one query (n slots, n a power of two up to 64*NR).
+//
+// Register form (LQ false): slot s sits in lane s % 64 of register s / 64
+// for the query's whole life.
+//
+// LDS form (LQ true): the beam is a queue of 64*NR slots in the wave's LDS
+// region, slot s at qd[s] / qi[s].  A merge loads the registers that hold
+// slots below n into the register form (load), runs the same networks and
+// stores them back (store); everything else (the expansion pick, the flag
+// ballots, the filter re-add, the output) reads the queue.  The beam then
+// holds no VGPRs across the distance gathers, where the kernel's register
+// count peaks (beam 512, d = 128: 127 VGPRs instead of 147, 4 waves per
+// SIMD instead of 3).  Both forms run the same comparisons in the same
+// order.
+template <int NR, bool LQ>
+struct Beam;
+
+template <int NR>
+struct Beam<NR, false> {
+  static constexpr bool kInLds = false;
+  float d[NR];
+  int i[NR];
+  __device__ __forceinline__ void init(int, int32_t npi, int) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      d[r] = kMaxD;
+      i[r] = npi;
+    }
+  }
+  // key of slot n - 1 (the beam's largest after a merge)
+  __device__ __forceinline__ float last(int n) const {
+    const int rl = (n - 1) >> 6, ll = (n - 1) & 63;
+    int v = 0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if (r == rl) v = __builtin_amdgcn_readlane(__float_as_int(d[r]), ll);
+    return __int_as_float(v);
+  }
+  // id of slot sl (uniform), which is marked expanded (-id)
+  __device__ __forceinline__ int take(int sl, int lane) {
+    int v = 0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      if ((sl >> 6) == r) {
+        v = __builtin_amdgcn_readlane(i[r], sl & 63);
+        if (lane == (sl & 63)) i[r] = -i[r];
+      }
+    return v;
+  }
+  __device__ __forceinline__ void ballots(int lane, int n, int64_t np, uint64_t (&bf)[NR],
+                                          uint64_t (&bz)[NR]) const {
+    flag_ballots<NR>(i, lane, n, np, bf, bz);
+  }
+  // slot r*64 + lane
+  __device__ __forceinline__ int id_at(int r, int) const { return i[r]; }
+  __device__ __forceinline__ float dist_at(int r, int) const { return d[r]; }
+};
+
+template <int NR>
+struct Beam<NR, true> {
+  static constexpr bool kInLds = true;
+  float* qd;
+  int* qi;
+  __device__ __forceinline__ void init(int n, int32_t npi, int lane) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      if (r * 64 >= n) continue;  // registers past the beam are never read
+      qd[r * 64 + lane] = kMaxD;
+      qi[r * 64 + lane] = npi;
+    }
+    wave_sync();
+  }
+  __device__ __forceinline__ float last(int n) const { return qd[n - 1]; }
+  __device__ __forceinline__ int take(int sl, int lane) {
+    const int v = __builtin_amdgcn_readfirstlane(qi[sl]);
+    if (lane == 0) qi[sl] = -v;
+    wave_sync();
+    return v;
+  }
+  __device__ __forceinline__ void ballots(int lane, int n, int64_t np, uint64_t (&bf)[NR],
+                                          uint64_t (&bz)[NR]) const {
+    int x[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) x[r] = r * 64 < n ? qi[r * 64 + lane] : 0;
+    flag_ballots<NR>(x, lane, n, np, bf, bz);
+  }
+  __device__ __forceinline__ int id_at(int r, int lane) const { return qi[r * 64 + lane]; }
+  __device__ __forceinline__ float dist_at(int r, int lane) const { return qd[r * 64 + lane]; }
+  __device__ __forceinline__ void load(int n, int lane, float (&d)[NR], int (&i)[NR]) const {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const bool in = r * 64 < n;  // uniform
+      d[r] = in ? qd[r * 64 + lane] : kMaxD;
+      i[r] = in ? qi[r * 64 + lane] : 0;
+    }
+  }
+  __device__ __forceinline__ void store(int n, int lane, const float (&d)[NR],
+                                        const int (&i)[NR]) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      if (r * 64 >= n) continue;
+      qd[r * 64 + lane] = d[r];
+      qi[r * 64 + lane] = i[r];
+    }
+    wave_sync();
+  }
 };
 
 
@@ -425,8 +564,7 @@ struct SearchCore {
   // B not below L[n-1] therefore loses its min-exchange, and with no swap the
   // merge is that second pass.  Distances are non-negative floats, so their
   // bit patterns order as unsigned integers.
-  static __device__ __forceinline__ bool batch_cannot_enter(const float (&ld)[NR],
-                                                            const float (&bd)[MR], int n,
+  static __device__ __forceinline__ bool batch_cannot_enter(float last, const float (&bd)[MR],
                                                             int lane) {
     int mn = -1;  // 0xffffffff: no valid element
 #pragma unroll
@@ -442,36 +580,25 @@ struct SearchCore {
     mn = (int)min((uint32_t)mn, (uint32_t)xor_lane<32>(mn, lane));
     const uint32_t bmin = (uint32_t)__builtin_amdgcn_readfirstlane(mn);
     if (bmin == 0xffffffffu) return true;
-    const int rl = (n - 1) >> 6, ll = (n - 1) & 63;
-    int lmax = 0;
-#pragma unroll
-    for (int r = 0; r < NR; ++r)
-      if (r == rl) lmax = __builtin_amdgcn_readlane(__float_as_int(ld[r]), ll);
     // REF swaps on a strictly larger distance; TUNED also on an equal
     // distance with a larger id, so only a strictly smaller L[n-1] is safe
-    return TUNED ? __int_as_float(lmax) < __uint_as_float(bmin)
-                 : __int_as_float(lmax) <= __uint_as_float(bmin);
+    return TUNED ? last < __uint_as_float(bmin) : last <= __uint_as_float(bmin);
   }
 
-  // sort B descending, min-exchange into L, merge L ascending
-  static __device__ __forceinline__ void merge_batch(const WaveLds& s, int lane, float (&ld)[NR],
-                                                     int (&li)[NR], float (&bd)[MR],
-                                                     int (&bi)[MR], int n, int m,
-                                                     bool skip_max) {
-    if (batch_cannot_enter(ld, bd, n, lane)) return;
-    sort_desc<TUNED, MR>(bd, bi, m, lane);
-#pragma unroll
-    for (int r = 0; r < MR; ++r) {
-      s.sbd[r * 64 + lane] = bd[r];
-      s.sbi[r * 64 + lane] = bi[r];
-    }
-    wave_sync();
+  // min-exchange of L's last c = min(n, m) slots with the sorted batch's
+  // last c (in LDS), then the merge of L
+  static __device__ __forceinline__ void exchange_merge(const WaveLds& s, int lane,
+                                                        float (&ld)[NR], int (&li)[NR], int n,
+                                                        int m, bool skip_max) {
     const int c = n < m ? n : m;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-      const int sl = r * 64 + lane;
-      if (sl >= n - c && sl < n) {
-        const int sb = sl - n + m;
+      // registers outside the exchanged slots [n - c, n) are skipped by a
+      // scalar branch (n and c are uniform); slot r*64 + lane is compared
+      // as lane against scalars, so no slot index stays live in a VGPR
+      if ((r + 1) * 64 <= n - c || r * 64 >= n) continue;
+      if (lane >= n - c - r * 64 && lane < n - r * 64) {
+        const int sb = lane + (r * 64 - n + m);
         const float xd = s.sbd[sb];
         const int xi = s.sbi[sb];
         if (!(skip_max && xd == kMaxD) && key_gt<TUNED>(ld[r], li[r], xd, xi)) {
@@ -483,20 +610,31 @@ struct SearchCore {
     wave_sync();
     merge_asc<TUNED, NR>(ld, li, n, lane);
   }
-};
 
-// Expansion-flag ballots of the beam (graph_search.cuh:1014-1031).
-template <int NR>
-__device__ __forceinline__ void flag_ballots(const int (&li)[NR], int lane, int n, int64_t np,
-                                             uint64_t (&bf)[NR], uint64_t (&bz)[NR]) {
+  // sort B descending, min-exchange into L, merge L ascending
+  template <class BeamT>
+  static __device__ __forceinline__ void merge_batch(const WaveLds& s, int lane, BeamT& L,
+                                                     float (&bd)[MR], int (&bi)[MR], int n, int m,
+                                                     bool skip_max) {
+    if (batch_cannot_enter(L.last(n), bd, lane)) return;
+    sort_desc<TUNED, MR>(bd, bi, m, lane);
 #pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    const int sl = r * 64 + lane;
-    const bool in = sl < n;
-    bf[r] = __ballot(in && li[r] > 0 && (int64_t)li[r] < np);
-    bz[r] = __ballot(in && li[r] == 0);
+    for (int r = 0; r < MR; ++r) {
+      s.sbd[r * 64 + lane] = bd[r];
+      s.sbi[r * 64 + lane] = bi[r];
+    }
+    wave_sync();
+    if constexpr (BeamT::kInLds) {
+      float ld[NR];
+      int li[NR];
+      L.load(n, lane, ld, li);
+      exchange_merge(s, lane, ld, li, n, m, skip_max);
+      L.store(n, lane, ld, li);
+    } else {
+      exchange_merge(s, lane, L.d, L.i, n, m, skip_max);
+    }
   }
-}
+};
 
 template <int NR>
 __device__ __forceinline__ uint32_t scan_chunk(const uint64_t (&bf)[NR], const uint64_t (&bz)[NR],
@@ -607,10 +745,11 @@ __device__ __forceinline__ float* dist_buf_t(const WaveLds& s, uint32_t* qshared
 #define dist_buf(s, q, par) dist_buf_t<MR, WPQ>(s, q, par)
 
 // LDS words of one wave's private region past the filter: compacted ids,
-// sorted batch (ids, dists) and, for one wave per query, the distances
-template <int MR, int WPQ>
+// sorted batch (ids, dists), for one wave per query the distances, and for
+// a beam held in LDS its queue of 64*NR (distance, id) slots
+template <int MR, int WPQ, int NR, bool LQ>
 __host__ __device__ constexpr int lds_words_wave() {
-  return (WPQ > 1 ? 3 : 4) * 64 * MR;
+  return (WPQ > 1 ? 3 : 4) * 64 * MR + (LQ ? 2 * 64 * NR : 0);
 }
 // LDS words a query shares between its waves (WPQ > 1): two distance
 // buffers (alternating batches) and the broadcast slot of the work queue
@@ -619,14 +758,14 @@ __host__ __device__ constexpr int lds_words_query() {
   return WPQ > 1 ? 2 * 64 * MR + 4 : 0;
 }
 
-template <bool TUNED, int NR, int MR, int J, int WPQ>
+template <bool TUNED, int NR, int MR, int J, int WPQ, bool LQ>
 __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
   using Core = SearchCore<TUNED, NR, MR, J, WPQ>;
   extern __shared__ uint32_t smem[];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int sub = wave % WPQ;  // this wave's share of the query's distances
-  const int region = a.bloom_words_max + lds_words_wave<MR, WPQ>();
+  const int region = a.bloom_words_max + lds_words_wave<MR, WPQ, NR, LQ>();
   uint32_t* base_lds = smem + wave * region;
   // shared by the query's waves (WPQ > 1; one query per workgroup)
   uint32_t* qshared = smem + (blockDim.x >> 6) * region;
@@ -669,6 +808,9 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
     s.sbd = (float*)(s.cid + 64 * MR);
     s.sbi = (int*)(s.sbd + 64 * MR);
     s.cdist = WPQ > 1 ? nullptr : (float*)(s.sbi + 64 * MR);
+    s.qd = LQ ? (float*)(base_lds + a.bloom_words_max + lds_words_wave<MR, WPQ, 0, false>())
+              : nullptr;
+    s.qi = LQ ? (int*)(s.qd + 64 * NR) : nullptr;
     // the hash count is fixed per mode (kHashes), checked on the host
     Bloom<kHashes<TUNED>> bl{s.bloom, (uint32_t)bwords, (uint32_t)(31 - __clz(bwords)),
                              (uint32_t)bwords * 32u};
@@ -683,13 +825,14 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
     bl.clear(lane);  // P1
     wave_sync();
 
-    float ld[NR], bd[MR];
-    int li[NR], bi[MR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      ld[r] = kMaxD;
-      li[r] = npi;
+    float bd[MR];
+    int bi[MR];
+    Beam<NR, LQ> L;
+    if constexpr (LQ) {
+      L.qd = s.qd;
+      L.qi = s.qi;
     }
+    L.init(n, npi, lane);
     int hit = 0;
     if (a.algo == 1) {
       hit = __builtin_amdgcn_readfirstlane(a.hits[q]);
@@ -721,7 +864,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
       }
       wave_sync();
       nE += Core::batch_dists(a, s, dist_buf(s, qshared, parity), qv, lane, sub, bi, bd, m);
-      Core::merge_batch(s, lane, ld, li, bd, bi, n, m, false);
+      Core::merge_batch(s, lane, L, bd, bi, n, m, false);
     }
 
     // ---- main loop, :964-1532
@@ -742,13 +885,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
       while (cnt < W && mask != 0u) {
         const int b = __builtin_ctz(mask);
         const int sl = chunk * 32 + b;
-        int v = 0;
-#pragma unroll
-        for (int r = 0; r < NR; ++r)
-          if ((sl >> 6) == r) {
-            v = __builtin_amdgcn_readlane(li[r], sl & 63);
-            if (lane == (sl & 63)) li[r] = -li[r];
-          }
+        const int v = L.take(sl, lane);
 #pragma unroll
         for (int x = 0; x < kMaxWidth; ++x)
           if (x == cnt) vrow[x] = v < 0 ? -v : v;
@@ -758,7 +895,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
         while (mask == 0u && chunk < nchunks) {
           ++chunk;
           uint64_t bf[NR], bz[NR];
-          flag_ballots<NR>(li, lane, n, np, bf, bz);
+          L.ballots(lane, n, np, bf, bz);
           mask = chunk < nchunks ? scan_chunk<NR>(bf, bz, chunk, czero) : 0u;
         }
       }
@@ -805,11 +942,11 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
       }
       nE += Core::batch_dists(a, s, dist_buf(s, qshared, parity), qv, lane, sub, bi, bd,
                               cnt << shift);
-      Core::merge_batch(s, lane, ld, li, bd, bi, n, m, true);
+      Core::merge_batch(s, lane, L, bd, bi, n, m, true);
       // rescan (:1495-1516)
       {
         uint64_t bf[NR], bz[NR];
-        flag_ballots<NR>(li, lane, n, np, bf, bz);
+        L.ballots(lane, n, np, bf, bz);
         for (chunk = 0; chunk < nchunks; ++chunk) {
           mask = scan_chunk<NR>(bf, bz, chunk, czero);
           if (mask != 0u) break;
@@ -821,9 +958,10 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
         wave_sync();
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-          const int sl = r * 64 + lane;
-          const int x = li[r] < 0 ? -li[r] : li[r];
-          bl.add((uint32_t)x, sl < n && (int64_t)x < np);
+          if (r * 64 >= n) continue;  // past the beam (uniform)
+          const int id = L.id_at(r, lane);
+          const int x = id < 0 ? -id : id;
+          bl.add((uint32_t)x, lane < n - r * 64 && (int64_t)x < np);
         }
         wave_sync();
         hash_it = 0;
@@ -835,10 +973,12 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
     const int64_t ob = q * (int64_t)a.k_stride;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
+      if (r * 64 >= k) continue;  // past the results (uniform k)
       const int sl = r * 64 + lane;
-      if (sub == 0 && sl < k) {
-        a.ids_out[ob + sl] = li[r] < 0 ? -li[r] : li[r];
-        if (a.dists_out) a.dists_out[ob + sl] = ld[r];
+      if (sub == 0 && lane < k - r * 64) {
+        const int id = L.id_at(r, lane);
+        a.ids_out[ob + sl] = id < 0 ? -id : id;
+        if (a.dists_out) a.dists_out[ob + sl] = L.dist_at(r, lane);
       }
     }
     if (a.stats_out && sub == 0 && lane == 0) {
@@ -850,40 +990,40 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
   }
 }
 
-template <int MR, int WPQ>
+template <int MR, int WPQ, int NR, bool LQ>
 size_t search_lds_bytes(int bloom_words_max, int wpb) {
-  return ((size_t)wpb * (bloom_words_max + lds_words_wave<MR, WPQ>()) +
+  return ((size_t)wpb * (bloom_words_max + lds_words_wave<MR, WPQ, NR, LQ>()) +
           lds_words_query<MR, WPQ>()) * 4;
 }
 
-template <bool TUNED, int NR, int MR, int J, int WPQ>
+template <bool TUNED, int NR, int MR, int J, int WPQ, bool LQ>
 hipError_t launch_t(const SearchArgs& a, int grid, int wpb, hipStream_t st, int* occ) {
-  const size_t lds = search_lds_bytes<MR, WPQ>(a.bloom_words_max, wpb);
+  const size_t lds = search_lds_bytes<MR, WPQ, NR, LQ>(a.bloom_words_max, wpb);
   if (occ) {  // resident workgroups per CU for this shape
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        occ, search_kernel<TUNED, NR, MR, J, WPQ>, 64 * wpb, lds);
+        occ, search_kernel<TUNED, NR, MR, J, WPQ, LQ>, 64 * wpb, lds);
   }
-  hipLaunchKernelGGL((search_kernel<TUNED, NR, MR, J, WPQ>), dim3(grid), dim3(64 * wpb), lds, st,
-                     a);
+  hipLaunchKernelGGL((search_kernel<TUNED, NR, MR, J, WPQ, LQ>), dim3(grid), dim3(64 * wpb), lds,
+                     st, a);
   return hipGetLastError();
 }
 
-template <int NR, int MR, int WPQ>
+template <int NR, int MR, int WPQ, bool LQ>
 hipError_t launch_nmw(const SearchArgs& a, int grid, int wpb, hipStream_t st, int* occ) {
   const int J = (a.d + 31) / 32;
   const bool tuned = a.mode != 0;
-#define RTE_J_CASES(T)                                                      \
-  switch (J) {                                                              \
-    case 1:                                                                 \
-    case 2: return launch_t<T, NR, MR, 2, WPQ>(a, grid, wpb, st, occ);      \
-    case 3: return launch_t<T, NR, MR, 3, WPQ>(a, grid, wpb, st, occ);      \
-    case 4: return launch_t<T, NR, MR, 4, WPQ>(a, grid, wpb, st, occ);      \
-    case 5:                                                                 \
-    case 6:                                                                 \
-    case 7:                                                                 \
-    case 8: return launch_t<T, NR, MR, 8, WPQ>(a, grid, wpb, st, occ);      \
-    case 30: return launch_t<T, NR, MR, 30, WPQ>(a, grid, wpb, st, occ);    \
-    default: return hipErrorInvalidValue;                                   \
+#define RTE_J_CASES(T)                                                        \
+  switch (J) {                                                                \
+    case 1:                                                                   \
+    case 2: return launch_t<T, NR, MR, 2, WPQ, LQ>(a, grid, wpb, st, occ);    \
+    case 3: return launch_t<T, NR, MR, 3, WPQ, LQ>(a, grid, wpb, st, occ);    \
+    case 4: return launch_t<T, NR, MR, 4, WPQ, LQ>(a, grid, wpb, st, occ);    \
+    case 5:                                                                   \
+    case 6:                                                                   \
+    case 7:                                                                   \
+    case 8: return launch_t<T, NR, MR, 8, WPQ, LQ>(a, grid, wpb, st, occ);    \
+    case 30: return launch_t<T, NR, MR, 30, WPQ, LQ>(a, grid, wpb, st, occ);  \
+    default: return hipErrorInvalidValue;                                     \
   }
   if (tuned) {
     RTE_J_CASES(true)

@@ -4,6 +4,6 @@
 #include "search_impl.h"
 
 namespace rte {
-template hipError_t launch_nmw<4, 1, 1>(const SearchArgs&, int, int, hipStream_t, int*);
-template hipError_t launch_nmw<4, 2, 1>(const SearchArgs&, int, int, hipStream_t, int*);
+template hipError_t launch_nmw<4, 1, 1, false>(const SearchArgs&, int, int, hipStream_t, int*);
+template hipError_t launch_nmw<4, 2, 1, false>(const SearchArgs&, int, int, hipStream_t, int*);
 }  // namespace rte

@@ -3,8 +3,8 @@
 #include "search_impl.h"
 
 namespace rte {
-template hipError_t launch_nmw<8, 1, 2>(const SearchArgs&, int, int, hipStream_t, int*);
-template hipError_t launch_nmw<8, 2, 2>(const SearchArgs&, int, int, hipStream_t, int*);
-template hipError_t launch_nmw<8, 1, 4>(const SearchArgs&, int, int, hipStream_t, int*);
-template hipError_t launch_nmw<8, 2, 4>(const SearchArgs&, int, int, hipStream_t, int*);
+template hipError_t launch_nmw<8, 1, 2, false>(const SearchArgs&, int, int, hipStream_t, int*);
+template hipError_t launch_nmw<8, 2, 2, false>(const SearchArgs&, int, int, hipStream_t, int*);
+template hipError_t launch_nmw<8, 1, 4, false>(const SearchArgs&, int, int, hipStream_t, int*);
+template hipError_t launch_nmw<8, 2, 4, false>(const SearchArgs&, int, int, hipStream_t, int*);
 }  // namespace rte

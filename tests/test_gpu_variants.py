@@ -1,5 +1,6 @@
 """GPU parity of the launch variants: two and four wavefronts per query
-(the query's waves split each batch's distances), the leaf-ordered batched
+(the query's waves split each batch's distances), the beam held in an LDS
+queue instead of VGPRs, the leaf-ordered batched
 queue (queries sorted by entry leaf on the device, one slot range per XCD),
 and the grid codes.  Every variant must give the oracle's ids, distances
 and statistics bit-for-bit: they change how the work is spread, never what
@@ -89,6 +90,55 @@ def test_multiwave_random_entry(ctx, small_workload):
         assert_same(sg, so.astype(np.int32), "stats")
 
 
+@pytest.mark.parametrize("beam,width,mode,wg", [(8, 1, 0, 4), (32, 1, 1, 4), (64, 2, 0, 1),
+                                                (128, 4, 1, 2), (256, 1, 0, 4), (256, 2, 1, 1),
+                                                (512, 1, 0, 4), (512, 1, 1, 2)])
+def test_lds_queue_search_parity(ctx, int_workload, beam, width, mode, wg):
+    """rte_search with the beam held in an LDS queue (RTE_QUEUE_LDS) equals
+    the oracle (ids, distances, statistics) for beams 8..512, one to four
+    rows per iteration, both modes and 1, 2 or 4 queries per workgroup."""
+    from rt_entry_amd import make_params
+    w = int_workload
+    _ctx_for(ctx, w)
+    print(f"[variant] queue=lds beam={beam} width={width} mode={mode} wg={wg}", flush=True)
+    k = min(10, beam)
+    io, do, so = _oracle(w, w.queries, w.hits, k=k, beam=beam, width=width, mode=mode)
+    for grid in (0, 5):
+        ig, dg, sg = ctx.search(w.queries, w.hits,
+                                make_params(k=k, beam=beam, width=width, mode=mode, queue=1,
+                                            wg_waves=wg, grid=grid), with_stats=True)
+        assert_same(ig, io, f"ids grid {grid}")
+        assert_same(dg.view(np.int32), do.view(np.int32), f"dists grid {grid}")
+        assert_same(sg, so.astype(np.int32), f"stats grid {grid}")
+
+
+def test_lds_queue_other_dims(ctx, small_workload, wide_workload, deg48_workload):
+    """The LDS queue at 960-d, a degree-48 graph, truncated d = 100 and
+    random (LCG) entries."""
+    from rt_entry_amd import make_params
+    for name, w, kw, d in (("960-d", wide_workload, dict(beam=128, max_iter=40), None),
+                           ("deg48", deg48_workload, dict(beam=256, width=2), None),
+                           ("d100", small_workload, dict(beam=128), 100),
+                           ("random entry", small_workload, dict(beam=256, algo=0), None)):
+        print(f"[variant] queue=lds {name}", flush=True)
+        if d is None:
+            _ctx_for(ctx, w)
+            base, q = w.base, w.queries
+        else:
+            ctx.set_base(w.base, d=d)
+            ctx.set_graph(w.graph_raw)
+            ctx.set_leaves(w.aabbs, w.pids)
+            base = np.ascontiguousarray(w.base[:, :d])
+            q = np.ascontiguousarray(w.queries[:, :d])
+        hits = None if kw.get("algo", 1) == 0 else w.hits
+        po = O.make_params(k=10, **kw)
+        io, do, so = O.search(base, w.graph, q, w.pids, hits, po, with_stats=True)
+        ig, dg, sg = ctx.search(q, hits, make_params(k=10, queue=1, **kw), with_stats=True)
+        assert_same(ig, io, f"{name} ids")
+        assert_same(dg.view(np.int32), do.view(np.int32), f"{name} dists")
+        assert_same(sg, so.astype(np.int32), f"{name} stats")
+
+
 def test_batched_leaf_order_and_wpq(ctx, int_workload):
     """Uniform batched launches with every (wpq, order, grid code) equal
     rte_search, over replays with eager work in between (the leaf sort's
@@ -104,11 +154,13 @@ def test_batched_leaf_order_and_wpq(ctx, int_workload):
     p = make_params(k=10, beam=64, max_iter=50)
     ref_i, ref_d = ctx.search(PQ, leaf, p)
     ref_i, ref_d = ref_i.cpu().numpy(), ref_d.cpu().numpy()
-    for wpq in (1, 2, 4):
+    for wpq, queue in ((1, 0), (2, 0), (4, 0), (1, 1)):
         for order in (0, 1):
             for grid in (0, -1, -2, -3, 3):
-                q = make_params(k=10, beam=64, max_iter=50, wpq=wpq, order=order, grid=grid)
-                print(f"[variant] batched wpq={wpq} order={order} grid={grid}", flush=True)
+                q = make_params(k=10, beam=64, max_iter=50, wpq=wpq, order=order, grid=grid,
+                                queue=queue)
+                print(f"[variant] batched wpq={wpq} queue={queue} order={order} grid={grid}",
+                      flush=True)
                 for rep in range(3):
                     ids, dists = ctx.search_batched(PQ, leaf, None, None, q, k_stride=10)
                     ctx.synchronize()
@@ -140,11 +192,12 @@ def test_batched_uneven_leaf_order(ctx, int_workload, mode):
     leaf = torch.from_numpy(w.hits).to(dev)
     kqd = torch.from_numpy(kq).to(dev)
     bqd = torch.from_numpy(bq).to(dev)
-    for wpq in (1, 2):
+    for wpq, queue in ((1, 0), (2, 0), (1, 1)):
         for rep in range(2):
-            print(f"[variant] uneven leaf order wpq={wpq} mode={mode} rep={rep}", flush=True)
-            ig, dg = ctx.search_batched(PQ, leaf, kqd, bqd, make_params(mode=mode, wpq=wpq,
-                                                                        order=1),
+            print(f"[variant] uneven leaf order wpq={wpq} queue={queue} mode={mode} rep={rep}",
+                  flush=True)
+            ig, dg = ctx.search_batched(PQ, leaf, kqd, bqd,
+                                        make_params(mode=mode, wpq=wpq, order=1, queue=queue),
                                         k_stride=io.shape[1])
             ctx.synchronize()
             ig, dg = ig.cpu().numpy(), dg.cpu().numpy()
@@ -167,9 +220,10 @@ def test_autotune_explores_new_knobs(ctx, int_workload):
     p = make_params(k=10, beam=128, max_iter=40)
     ref_i, _ = ctx.search(PQ, leaf, p)
     best, ms = ctx.autotune_batched(PQ, leaf, None, None, p, k_stride=10, bloom_words=(0, 1024))
-    print(f"[variant] autotune wpq={best.wpq} wg={best.wg_waves} grid={best.grid} "
-          f"order={best.order} bloom={best.bloom_words} {ms:.3f} ms", flush=True)
+    print(f"[variant] autotune wpq={best.wpq} wg={best.wg_waves} queue={best.queue} "
+          f"grid={best.grid} order={best.order} bloom={best.bloom_words} {ms:.3f} ms", flush=True)
     assert best.wpq in (1, 2, 4) and best.order in (0, 1) and -3 <= best.grid <= 0
+    assert best.queue in (0, 1) and not (best.queue == 1 and best.wpq > 1)
     assert best.bloom_words in (0, 1024) and ms > 0
     ids, _ = ctx.search_batched(PQ, leaf, None, None, best, k_stride=10)
     ctx.synchronize()
@@ -181,3 +235,7 @@ def test_autotune_explores_new_knobs(ctx, int_workload):
         ctx.search(PQ, leaf, make_params(k=10, beam=64, wpq=2, wg_waves=4))
     with pytest.raises(RteError):
         ctx.search_batched(PQ, leaf, None, None, make_params(k=10, beam=64, order=2))
+    with pytest.raises(RteError):
+        ctx.search(PQ, leaf, make_params(k=10, beam=64, queue=2))
+    with pytest.raises(RteError):  # the LDS queue runs one wave per query
+        ctx.search(PQ, leaf, make_params(k=10, beam=64, queue=1, wpq=2))

@@ -46,16 +46,17 @@ def test_noalloc_matches_batched(ctx, int_workload):
     bq = torch.from_numpy(bq_h).to(dev)
     for uneven in (False, True):
         for order in (0, 1):
-            for wpq in (1, 2):
+            for wpq, queue in ((1, 0), (2, 0), (1, 1)):
                 if uneven:
-                    p = make_params(mode=0, order=order, wpq=wpq)
+                    p = make_params(mode=0, order=order, wpq=wpq, queue=queue)
                     k_stride = int(kq_h.max())
                 else:
-                    p = make_params(k=10, beam=64, max_iter=50, order=order, wpq=wpq)
+                    p = make_params(k=10, beam=64, max_iter=50, order=order, wpq=wpq,
+                                    queue=queue)
                     k_stride = 10
                 # two calls in a row, the second on a smaller batch
                 for n in (nq, nq // 3):
-                    what = f"uneven={uneven} order={order} wpq={wpq} nq={n}"
+                    what = f"uneven={uneven} order={order} wpq={wpq} queue={queue} nq={n}"
                     print(f"[noalloc] {what}", flush=True)
                     args = (PQ[:n], leaf[:n], kq[:n] if uneven else None,
                             bq[:n] if uneven else None)
