@@ -219,7 +219,8 @@ int rte_synchronize(rte_ctx* c) {
 
 int rte_set_base(rte_ctx* c, const float* base, int64_t np, int d, int64_t stride, int where) {
   if (!c) return RTE_EINVAL;
-  if (!base || np <= 0 || np >= 0x7fffffff || d <= 0 || stride < d)
+  // the search forms row byte offsets as 32-bit id x 32-bit row bytes
+  if (!base || np <= 0 || np >= 0x7fffffff || d <= 0 || stride < d || stride >= (1 << 30))
     return fail(c, RTE_EINVAL, "bad base shape np=%lld d=%d stride=%lld", (long long)np, d,
                 (long long)stride);
   if (d > 960) return fail(c, RTE_EUNSUP, "d=%d above 960", d);

@@ -19,7 +19,8 @@
 //    VALU operations with the same pairings;
 //  - neighbour ids that pass the filter are compacted before the gathers, so
 //    one iteration issues its loads together and only for points that need a
-//    distance.
+//    distance; a lane reads its half-wave's ids in 16-byte LDS reads and
+//    forms each row address with one 32x32->64 multiply-add.
 // Parity decisions P1-P7 match the CPU oracle (zeroed filter, all-ones first
 // mask, snapshot filter test per row, no row read for ids >= np,
 // barrier-separated substeps, REF/TUNED comparators).
@@ -486,6 +487,63 @@ struct SearchCore {
   static constexpr int PB = PB0 / WPQ > 0 ? PB0 / WPQ : 1;
   static constexpr int SH = 5 - ilog2c(PB);  // a point's lanes: l32 >> SH
 
+  // The PB ids of one load group for this lane's half-wave.  Compacted
+  // point kk sits at cid[(kk & 1) * 32*MR + kk / 2], so the points
+  // p0 + 2*pb + half of a group are consecutive words and arrive in 16-byte
+  // LDS reads.
+  static __device__ __forceinline__ void group_ids(const int* c, int (&ids)[PB]) {
+    if constexpr (PB >= 4) {
+#pragma unroll
+      for (int i = 0; i < PB / 4; ++i) {
+        const int4 x = reinterpret_cast<const int4*>(c)[i];
+        ids[4 * i] = x.x;
+        ids[4 * i + 1] = x.y;
+        ids[4 * i + 2] = x.z;
+        ids[4 * i + 3] = x.w;
+      }
+    } else if constexpr (PB == 2) {
+      const int2 x = reinterpret_cast<const int2*>(c)[0];
+      ids[0] = x.x;
+      ids[1] = x.y;
+    } else {
+      ids[0] = c[0];
+    }
+  }
+
+  // Row values of the group: lane l32 of a half-wave loads dims l32 + 32j
+  // of its half's points, from a 64-bit row address made by one 32x32->64
+  // multiply-add.  The launcher picks J = ceil(d / 32) for J = 3, 4 and 30,
+  // so there only the last dim group can pass d: a lane whose last dim is
+  // past d reads its row's first dim instead (in bounds) and takes 0, with
+  // no branch.  J = 2 and 8 also serve smaller d (1..32, 129..224): every
+  // dim group is masked there.
+  static __device__ __forceinline__ void group_rows(const SearchArgs& a, const int (&ids)[PB],
+                                                    int l32, float (&v)[PB][J]) {
+    const char* bp = reinterpret_cast<const char*>(a.base) + 4 * l32;
+    const uint32_t rowb = (uint32_t)a.stride * 4u;
+    if constexpr (J == 2 || J == 8) {
+#pragma unroll
+      for (int pb = 0; pb < PB; ++pb) {
+        const float* row = reinterpret_cast<const float*>(
+            bp + (uint64_t)(uint32_t)ids[pb] * (uint64_t)rowb);
+#pragma unroll
+        for (int j = 0; j < J; ++j) v[pb][j] = (l32 + 32 * j) < a.d ? row[32 * j] : 0.0f;
+      }
+    } else {
+      const bool last_in = (l32 + 32 * (J - 1)) < a.d;
+      const int lastj = last_in ? 32 * (J - 1) : 0;
+#pragma unroll
+      for (int pb = 0; pb < PB; ++pb) {
+        const float* row = reinterpret_cast<const float*>(
+            bp + (uint64_t)(uint32_t)ids[pb] * (uint64_t)rowb);
+#pragma unroll
+        for (int j = 0; j < J - 1; ++j) v[pb][j] = row[32 * j];
+        const float x = row[lastj];
+        v[pb][J - 1] = last_in ? x : 0.0f;
+      }
+    }
+  }
+
   // distances of the valid ids of B (id in [0,np)); others get MAXD.
   // Every wave of the query compacts the same ids (its own copy); wave `sub`
   // computes the load groups sub, sub + WPQ, ... into cdist (shared by the
@@ -495,6 +553,7 @@ struct SearchCore {
                                                     float* cdist, const float (&qv)[J],
                                                     int lane, int sub, const int (&bi)[MR],
                                                     float (&bd)[MR], int m) {
+    constexpr int HALF = 32 * MR;  // cid words per half-wave
     const int half = lane >> 5, l32 = lane & 31;
     int total = 0;
     int cix[MR];
@@ -506,29 +565,24 @@ struct SearchCore {
       const int ci = total + (int)__builtin_amdgcn_mbcnt_hi(
                                  (uint32_t)(bal >> 32),
                                  __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-      if (valid) s.cid[ci] = bi[r];
+      if (valid) s.cid[(ci & 1) * HALF + (ci >> 1)] = bi[r];
       cix[r] = valid ? ci : -1;
       total += __popcll(bal);
     }
+    // the last group is padded to 2*PB points with row 0 (always a valid
+    // row; its distances are discarded)
+    const int padded = (total + 2 * PB - 1) & ~(2 * PB - 1);
+    if (lane < padded - total) {
+      const int kk = total + lane;
+      s.cid[(kk & 1) * HALF + (kk >> 1)] = 0;
+    }
     wave_sync();
-    const bool full = a.d == 32 * J;
-    const char* basep = reinterpret_cast<const char*>(a.base) + 4 * l32;
-    const int64_t rowb = a.stride * 4;
+    const int* cbase = s.cid + half * HALF;
     for (int p0 = 2 * PB * sub; p0 < total; p0 += 2 * PB * WPQ) {
+      int ids[PB];
+      group_ids(cbase + (p0 >> 1), ids);
       float v[PB][J];
-#pragma unroll
-      for (int pb = 0; pb < PB; ++pb) {
-        // groups past the end reload the last point (finite, discarded)
-        const int kk = min(p0 + 2 * pb + half, total - 1);
-        const float* row = reinterpret_cast<const float*>(basep + (int64_t)s.cid[kk] * rowb);
-        if (full) {
-#pragma unroll
-          for (int j = 0; j < J; ++j) v[pb][j] = row[32 * j];
-        } else {
-#pragma unroll
-          for (int j = 0; j < J; ++j) v[pb][j] = (l32 + 32 * j) < a.d ? row[32 * j] : 0.0f;
-        }
-      }
+      group_rows(a, ids, l32, v);
       float part[PB];
 #pragma unroll
       for (int pb = 0; pb < PB; ++pb) {
@@ -744,6 +798,10 @@ __device__ __forceinline__ float* dist_buf_t(const WaveLds& s, uint32_t* qshared
 }
 #define dist_buf(s, q, par) dist_buf_t<MR, WPQ>(s, q, par)
 
+// LDS words reserved for a filter of w words: a multiple of four, so the
+// compacted ids after it are 16-byte aligned for vector reads
+__host__ __device__ constexpr int filter_words(int w) { return (w + 3) & ~3; }
+
 // LDS words of one wave's private region past the filter: compacted ids,
 // sorted batch (ids, dists), for one wave per query the distances, and for
 // a beam held in LDS its queue of 64*NR (distance, id) slots
@@ -765,7 +823,8 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int sub = wave % WPQ;  // this wave's share of the query's distances
-  const int region = a.bloom_words_max + lds_words_wave<MR, WPQ, NR, LQ>();
+  const int fwords = filter_words(a.bloom_words_max);
+  const int region = fwords + lds_words_wave<MR, WPQ, NR, LQ>();
   uint32_t* base_lds = smem + wave * region;
   // shared by the query's waves (WPQ > 1; one query per workgroup)
   uint32_t* qshared = smem + (blockDim.x >> 6) * region;
@@ -804,11 +863,11 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
     bwords = __builtin_amdgcn_readfirstlane(bwords);
     WaveLds s;
     s.bloom = base_lds;
-    s.cid = (int*)(base_lds + a.bloom_words_max);
+    s.cid = (int*)(base_lds + fwords);
     s.sbd = (float*)(s.cid + 64 * MR);
     s.sbi = (int*)(s.sbd + 64 * MR);
     s.cdist = WPQ > 1 ? nullptr : (float*)(s.sbi + 64 * MR);
-    s.qd = LQ ? (float*)(base_lds + a.bloom_words_max + lds_words_wave<MR, WPQ, 0, false>())
+    s.qd = LQ ? (float*)(base_lds + fwords + lds_words_wave<MR, WPQ, 0, false>())
               : nullptr;
     s.qi = LQ ? (int*)(s.qd + 64 * NR) : nullptr;
     // the hash count is fixed per mode (kHashes), checked on the host
@@ -992,7 +1051,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
 
 template <int MR, int WPQ, int NR, bool LQ>
 size_t search_lds_bytes(int bloom_words_max, int wpb) {
-  return ((size_t)wpb * (bloom_words_max + lds_words_wave<MR, WPQ, NR, LQ>()) +
+  return ((size_t)wpb * (filter_words(bloom_words_max) + lds_words_wave<MR, WPQ, NR, LQ>()) +
           lds_words_query<MR, WPQ>()) * 4;
 }
 
