@@ -73,6 +73,17 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def fixed_shape(p, spec: str):
+    """p with the batched launch shape 'WPQ,WG_WAVES,QUEUE,GRID,ORDER' (the
+    autotuner's knobs), for runs that must launch one known shape, such as
+    counter runs, where the autotuner's timings are distorted."""
+    from rt_entry_amd import make_params
+    wpq, wg, queue, grid, order = (int(x) for x in spec.split(","))
+    return make_params(k=p.k, beam=p.beam, width=p.width, max_iter=p.max_iter, mode=p.mode,
+                       algo=p.algo, bloom_words=p.bloom_words, bloom_hashes=p.bloom_hashes,
+                       wg_waves=wg, grid=grid, wpq=wpq, order=order, queue=queue)
+
+
 def prepare(workload: int, scale: float, dev, n_train: int = 0, keep_raw: bool = False,
             entry_links: tuple = (0, 0)):
     import torch
@@ -266,8 +277,12 @@ def bench_uneven(args):
     p0 = make_params(k=kmax, beam=256, max_iter=max_iter, mode=args.mode)
     ctx.project(W["Q"], out=pq)
     ctx.select_entry(pq, out=hits)
-    tuned, tms = ctx.autotune_batched(pq, hits, kd_t, bd_t, p0, k_stride=kmax)
-    log(f"autotune: {launch_desc(tuned)} {tms:.3f} ms")
+    if args.shape:
+        tuned = fixed_shape(p0, args.shape)
+        log(f"fixed shape: {launch_desc(tuned)}")
+    else:
+        tuned, tms = ctx.autotune_batched(pq, hits, kd_t, bd_t, p0, k_stride=kmax)
+        log(f"autotune: {launch_desc(tuned)} {tms:.3f} ms")
     # per-class subsets (gathered inside the per-class step, as a caller
     # without the batched launch would have to)
     classes = sorted(set(zip(k_q.tolist(), b_q.tolist())))
@@ -518,6 +533,9 @@ def main():
                     help="time every launch variant (waves per query, workgroup, order, grid) "
                          "of the search with kd-tree entries instead of the benchmark")
     ap.add_argument("--nq", type=int, default=0, help="--launch-table: first n queries only")
+    ap.add_argument("--shape", default="",
+                    help="WPQ,WG_WAVES,QUEUE,GRID,ORDER: batched launch shape instead of the "
+                         "autotuner's choice (counter runs)")
     ap.add_argument("--verbose", action="store_true")
     args = ap.parse_args()
 
@@ -644,7 +662,7 @@ def main():
     beams = (args.beam,) if args.beam else (128, 256, 512) if mixed else (32, 64, 128, 256)
     iters = (args.max_iter or 100,) if args.beam else (10, 15, 20, 25, 30, 35, 40, 45, 50, 55, 60,
                                                        70, 80, 100, 150, 200) + (
-        (250, 300) if mixed else ())
+        (250, 300, 400, 500) if mixed else ())
 
     def setting(spec, name):
         """Fixed (beam, max_iter) from 'B,I', or the sweep."""
@@ -774,12 +792,16 @@ def main():
             ctx.project(W["Q"], out=pq)
             entry0 = entries(use)  # entries for the tuning batch
             pt = with_k(p, best[1]) if sd else p
-            if mixed:
+            if args.shape:
+                tuned = fixed_shape(p if mixed else pt, args.shape)
+                log(f"fixed shape: {launch_desc(tuned)}")
+            elif mixed:
                 tuned, tms = ctx.autotune_batched(pq, entry0, kq_d, beams_of(p.beam), p,
                                                   k_stride=k_out)
+                log(f"autotune: {launch_desc(tuned)} {tms:.3f} ms")
             else:
                 tuned, tms = ctx.autotune_batched(pq, entry0, None, None, pt, k_stride=pt.k)
-            log(f"autotune: {launch_desc(tuned)} {tms:.3f} ms")
+                log(f"autotune: {launch_desc(tuned)} {tms:.3f} ms")
 
         def run_search(entry, stats=None):
             if batched and mixed:

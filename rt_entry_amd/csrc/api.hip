@@ -635,14 +635,83 @@ __global__ void key_hist_kernel(KeySort k) {
   if (q < k.nq) atomicAdd(&k.hist[sort_key(k, q)], 1);
 }
 
-// one workgroup of 1024 threads: thread t scans keys [t*per, (t+1)*per)
+// One workgroup of 1024 threads: exclusive prefix sums of the histogram into
+// the cursors, in passes of kScanChunk keys staged in LDS.  A pass loads its
+// keys with coalesced reads (all issued before any store, which would make
+// each later load wait), thread t sums the consecutive keys
+// [t*per, (t+1)*per) of the pass, a workgroup scan combines the sums, and
+// the prefixes leave with coalesced writes that also zero the histogram.
+constexpr int kScanChunk = 8192;
 __global__ __launch_bounds__(1024) void key_scan_kernel(KeySort k) {
+  __shared__ int32_t vals[kScanChunk];
   __shared__ int32_t part[2][1024];
   const int t = threadIdx.x;
+  int32_t carry = 0;  // keys of the earlier passes
+  for (int c0 = 0; c0 < k.n_keys; c0 += kScanChunk) {
+    const int nc = min(kScanChunk, k.n_keys - c0);
+    for (int i = t; i < nc; i += 1024) vals[i] = k.hist[c0 + i];
+    __syncthreads();
+    const int per = (nc + 1023) / 1024;
+    const int k0 = min(nc, t * per), k1 = min(nc, k0 + per);
+    int32_t sum = 0;
+    for (int i = k0; i < k1; ++i) sum += vals[i];
+    part[0][t] = sum;
+    __syncthreads();
+    int src = 0;
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+      const int32_t v = part[src][t] + (t >= off ? part[src][t - off] : 0);
+      part[src ^ 1][t] = v;
+      __syncthreads();
+      src ^= 1;
+    }
+    int32_t run = carry + part[src][t] - sum;  // exclusive prefix of this thread's keys
+    for (int i = k0; i < k1; ++i) {
+      const int32_t h = vals[i];
+      vals[i] = run;
+      run += h;
+    }
+    carry += part[src][1023];
+    __syncthreads();
+    for (int i = t; i < nc; i += 1024) {
+      k.cursor[c0 + i] = vals[i];
+      k.hist[c0 + i] = 0;
+    }
+    __syncthreads();  // vals and part are rewritten by the next pass
+  }
+  if (t < 16 * k.segments) k.seg[t] = 0;
+}
+
+// The whole sort in one workgroup for key sets that fit in LDS: histogram by
+// LDS atomics, the same exclusive scan in place, then the scatter by LDS
+// atomics on the scanned counts.  Each thread reads the keys of up to
+// kSortPer queries of a pass before it uses any, so a pass costs one memory
+// round trip.  Replaces the three kernels below (and their launch gaps) for
+// n_keys <= kSortMaxKeys; leaves the global histogram untouched (zero).
+constexpr int kSortMaxKeys = 24576;
+constexpr int kSortPer = 16;
+__global__ __launch_bounds__(1024) void key_sort_one_kernel(KeySort k) {
+  __shared__ int32_t h[kSortMaxKeys];
+  __shared__ int32_t part[2][1024];
+  const int t = threadIdx.x;
+  for (int i = t; i < k.n_keys; i += 1024) h[i] = 0;
+  __syncthreads();
+  const int64_t pass = 1024 * kSortPer;
+  for (int64_t q0 = 0; q0 < k.nq; q0 += pass) {
+    int key[kSortPer];
+#pragma unroll
+    for (int j = 0; j < kSortPer; ++j) {
+      const int64_t q = q0 + 1024 * j + t;
+      key[j] = q < k.nq ? sort_key(k, q) : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kSortPer; ++j)
+      if (key[j] >= 0) atomicAdd(&h[key[j]], 1);
+  }
+  __syncthreads();
   const int per = (k.n_keys + 1023) / 1024;
-  const int k0 = t * per, k1 = min(k.n_keys, k0 + per);
+  const int k0 = min(k.n_keys, t * per), k1 = min(k.n_keys, k0 + per);
   int32_t sum = 0;
-  for (int i = k0; i < k1; ++i) sum += k.hist[i];
+  for (int i = k0; i < k1; ++i) sum += h[i];
   part[0][t] = sum;
   __syncthreads();
   int src = 0;
@@ -652,12 +721,23 @@ __global__ __launch_bounds__(1024) void key_scan_kernel(KeySort k) {
     __syncthreads();
     src ^= 1;
   }
-  int32_t run = part[src][t] - sum;  // exclusive prefix of this thread's range
+  int32_t run = part[src][t] - sum;
   for (int i = k0; i < k1; ++i) {
-    const int32_t h = k.hist[i];
-    k.cursor[i] = run;
-    k.hist[i] = 0;
-    run += h;
+    const int32_t c = h[i];
+    h[i] = run;
+    run += c;
+  }
+  __syncthreads();
+  for (int64_t q0 = 0; q0 < k.nq; q0 += pass) {
+    int key[kSortPer];
+#pragma unroll
+    for (int j = 0; j < kSortPer; ++j) {
+      const int64_t q = q0 + 1024 * j + t;
+      key[j] = q < k.nq ? sort_key(k, q) : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kSortPer; ++j)
+      if (key[j] >= 0) k.order[atomicAdd(&h[key[j]], 1)] = (int32_t)(q0 + 1024 * j + t);
   }
   if (t < 16 * k.segments) k.seg[t] = 0;
 }
@@ -724,9 +804,13 @@ static hipError_t enqueue_batched(rte_ctx* c, const rte::SearchArgs& a, const Ba
     if (bp.k_in)
       hipLaunchKernelGGL(batch_prep_kernel, dim3(nb), dim3(256), 0, c->stream, bp,
                          (int32_t*)a.k_per_q, (int32_t*)a.beam_per_q, (int64_t*)nullptr);
-    hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
-    hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(1024), 0, c->stream, *ks);
-    hipLaunchKernelGGL(key_scatter_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
+    if (ks->n_keys <= kSortMaxKeys) {
+      hipLaunchKernelGGL(key_sort_one_kernel, dim3(1), dim3(1024), 0, c->stream, *ks);
+    } else {
+      hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
+      hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(1024), 0, c->stream, *ks);
+      hipLaunchKernelGGL(key_scatter_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
+    }
   }
   const hipError_t e1 = hipGetLastError();
   if (e1 != hipSuccess) return e1;
