@@ -681,67 +681,6 @@ __global__ __launch_bounds__(1024) void key_scan_kernel(KeySort k) {
   if (t < 16 * k.segments) k.seg[t] = 0;
 }
 
-// The whole sort in one workgroup for key sets that fit in LDS: histogram by
-// LDS atomics, the same exclusive scan in place, then the scatter by LDS
-// atomics on the scanned counts.  Each thread reads the keys of up to
-// kSortPer queries of a pass before it uses any, so a pass costs one memory
-// round trip.  Replaces the three kernels below (and their launch gaps) for
-// n_keys <= kSortMaxKeys; leaves the global histogram untouched (zero).
-constexpr int kSortMaxKeys = 24576;
-constexpr int kSortPer = 16;
-__global__ __launch_bounds__(1024) void key_sort_one_kernel(KeySort k) {
-  __shared__ int32_t h[kSortMaxKeys];
-  __shared__ int32_t part[2][1024];
-  const int t = threadIdx.x;
-  for (int i = t; i < k.n_keys; i += 1024) h[i] = 0;
-  __syncthreads();
-  const int64_t pass = 1024 * kSortPer;
-  for (int64_t q0 = 0; q0 < k.nq; q0 += pass) {
-    int key[kSortPer];
-#pragma unroll
-    for (int j = 0; j < kSortPer; ++j) {
-      const int64_t q = q0 + 1024 * j + t;
-      key[j] = q < k.nq ? sort_key(k, q) : -1;
-    }
-#pragma unroll
-    for (int j = 0; j < kSortPer; ++j)
-      if (key[j] >= 0) atomicAdd(&h[key[j]], 1);
-  }
-  __syncthreads();
-  const int per = (k.n_keys + 1023) / 1024;
-  const int k0 = min(k.n_keys, t * per), k1 = min(k.n_keys, k0 + per);
-  int32_t sum = 0;
-  for (int i = k0; i < k1; ++i) sum += h[i];
-  part[0][t] = sum;
-  __syncthreads();
-  int src = 0;
-  for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
-    const int32_t v = part[src][t] + (t >= off ? part[src][t - off] : 0);
-    part[src ^ 1][t] = v;
-    __syncthreads();
-    src ^= 1;
-  }
-  int32_t run = part[src][t] - sum;
-  for (int i = k0; i < k1; ++i) {
-    const int32_t c = h[i];
-    h[i] = run;
-    run += c;
-  }
-  __syncthreads();
-  for (int64_t q0 = 0; q0 < k.nq; q0 += pass) {
-    int key[kSortPer];
-#pragma unroll
-    for (int j = 0; j < kSortPer; ++j) {
-      const int64_t q = q0 + 1024 * j + t;
-      key[j] = q < k.nq ? sort_key(k, q) : -1;
-    }
-#pragma unroll
-    for (int j = 0; j < kSortPer; ++j)
-      if (key[j] >= 0) k.order[atomicAdd(&h[key[j]], 1)] = (int32_t)(q0 + 1024 * j + t);
-  }
-  if (t < 16 * k.segments) k.seg[t] = 0;
-}
-
 __global__ void key_scatter_kernel(KeySort k) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < k.nq) k.order[atomicAdd(&k.cursor[sort_key(k, q)], 1)] = (int32_t)q;
@@ -804,13 +743,9 @@ static hipError_t enqueue_batched(rte_ctx* c, const rte::SearchArgs& a, const Ba
     if (bp.k_in)
       hipLaunchKernelGGL(batch_prep_kernel, dim3(nb), dim3(256), 0, c->stream, bp,
                          (int32_t*)a.k_per_q, (int32_t*)a.beam_per_q, (int64_t*)nullptr);
-    if (ks->n_keys <= kSortMaxKeys) {
-      hipLaunchKernelGGL(key_sort_one_kernel, dim3(1), dim3(1024), 0, c->stream, *ks);
-    } else {
-      hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
-      hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(1024), 0, c->stream, *ks);
-      hipLaunchKernelGGL(key_scatter_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
-    }
+    hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
+    hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(1024), 0, c->stream, *ks);
+    hipLaunchKernelGGL(key_scatter_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
   }
   const hipError_t e1 = hipGetLastError();
   if (e1 != hipSuccess) return e1;
