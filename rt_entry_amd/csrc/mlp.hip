@@ -33,17 +33,71 @@ __device__ __host__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// ---- v_mfma_f32_16x16x32_bf16 on gfx950 (training GEMMs and inference) ----
+// Operand layout of the 16x16x32 MFMA used here: lane l supplies row (A) or
+// column (B) l % 16 and the 8 consecutive k values 8 * (l / 16) + 0..7 of a
+// 32-wide k step (A and B use the same k assignment, so only the order of
+// the k sum is implied); lane l holds D[4 * (l / 16) + v][l % 16], v = 0..3.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ floatx4 mfma_k32(uint4 a, uint4 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
 enum Epi { EPI_RELU = 0, EPI_LINEAR = 1, EPI_RELU_BWD = 2, EPI_STORE = 3 };
+
+// 64 x 32 (rows x k) tile of a bf16 matrix whose element (row, k) sits at
+// P[row * srow + k * sk], for LDS T[64][40] ([row][k], rows 80 bytes apart so
+// every 16-byte MFMA operand read is aligned).  KC: k is the contiguous
+// dimension (sk == 1); thread t takes the 8 k values 8 * (t & 3) + 0..7 of
+// row t >> 2 as one 16-byte load.  Otherwise rows are contiguous (srow ==
+// 1); thread t takes rows 8 * (t & 7) + 0..7 of k value t >> 3 as one
+// 16-byte load and spreads them over 8 LDS rows.  Runs that cross the
+// matrix edge, or strides that break 16-byte alignment (vec false), are
+// loaded element by element with zeros outside.
+template <bool KC>
+__device__ __forceinline__ uint4 tile_load(const uint16_t* __restrict__ P, int64_t srow,
+                                           int64_t sk, int R, int K, int r0, int k0, int t,
+                                           bool vec) {
+  const int r = KC ? r0 + (t >> 2) : r0 + 8 * (t & 7);
+  const int k = KC ? k0 + 8 * (t & 3) : k0 + (t >> 3);
+  const bool full = KC ? (r < R && k + 8 <= K) : (r + 8 <= R && k < K);
+  if (vec && full) return *reinterpret_cast<const uint4*>(P + (int64_t)r * srow + (int64_t)k * sk);
+  uint16_t e[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int ri = KC ? r : r + i, ki = KC ? k + i : k;
+    e[i] = (ri < R && ki < K) ? P[(int64_t)ri * srow + (int64_t)ki * sk] : (uint16_t)0;
+  }
+  uint4 v;
+  memcpy(&v, e, 16);
+  return v;
+}
+template <bool KC>
+__device__ __forceinline__ void tile_store(uint16_t (*T)[40], int t, uint4 v) {
+  if constexpr (KC) {
+    *reinterpret_cast<uint4*>(&T[t >> 2][8 * (t & 3)]) = v;
+  } else {
+    uint16_t e[8];
+    memcpy(e, &v, 16);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) T[8 * (t & 7) + i][t >> 3] = e[i];
+  }
+}
 
 // C[M x N] = op(A)[M x K] . op(B)[K x N] with bf16 operands addressed by
 // strides (element (m,k) of A at A[m*sam + k*sak], (k,n) of B at
-// B[k*sbk + n*sbn]).  Workgroup tile 64x64, four waves of 32x32 (2x2 tiles of
-// v_mfma_f32_16x16x16_bf16), K staged through LDS 32 at a time.  Epilogues:
+// B[k*sbk + n*sbn]); AK / BK: k is A's / B's contiguous dimension (else m /
+// n is).  Workgroup tile 64x64, four waves of 32x32 (2x2 tiles of
+// v_mfma_f32_16x16x32_bf16), K staged through LDS 32 at a time with 16-byte
+// loads, the next K tile's loads in flight while the current one is
+// multiplied.  Epilogues:
 //  RELU      out = relu(acc + bias[n]) (f32) and its bf16 copy
 //  LINEAR    out = acc + bias[n]
 //  RELU_BWD  out = acc * (act[m,n] > 0) (f32) and its bf16 copy
 //  STORE     out = acc
-template <int EPI>
+template <int EPI, bool AK, bool BK>
 __global__ __launch_bounds__(256) void gemm_bf16(const uint16_t* __restrict__ A, int64_t sam,
                                                  int64_t sak, const uint16_t* __restrict__ B,
                                                  int64_t sbk, int64_t sbn, int M, int N, int K,
@@ -51,51 +105,49 @@ __global__ __launch_bounds__(256) void gemm_bf16(const uint16_t* __restrict__ A,
                                                  const float* __restrict__ act,
                                                  float* __restrict__ out,
                                                  uint16_t* __restrict__ out16) {
-  __shared__ uint16_t As[64][32 + 4];  // [m][k]
-  __shared__ uint16_t Bs[64][32 + 4];  // [n][k]
+  __shared__ __attribute__((aligned(16))) uint16_t As[64][40];  // [m][k]
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[64][40];  // [n][k]
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  // 16-byte loads need the assumed contiguous stride and 16-byte alignment
+  const bool va = ((uintptr_t)A & 15) == 0 && (AK ? sak == 1 && (sam & 7) == 0
+                                                  : sam == 1 && (sak & 7) == 0);
+  const bool vb = ((uintptr_t)B & 15) == 0 && (BK ? sbk == 1 && (sbn & 7) == 0
+                                                  : sbn == 1 && (sbk & 7) == 0);
   floatx4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  uint4 ra = tile_load<AK>(A, sam, sak, M, K, m0, 0, t, va);
+  uint4 rb = tile_load<BK>(B, sbn, sbk, N, K, n0, 0, t, vb);
+  tile_store<AK>(As, t, ra);
+  tile_store<BK>(Bs, t, rb);
+  __syncthreads();
+  const int r = lane & 15, kq = 8 * (lane >> 4);
   for (int k0 = 0; k0 < K; k0 += 32) {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int idx = t + 256 * e;  // 0..2047 over a 64x32 tile
-      const int r = idx >> 5, c = idx & 31;
-      const int gm = m0 + r, gk = k0 + c;
-      As[r][c] = (gm < M && gk < K) ? A[(int64_t)gm * sam + (int64_t)gk * sak] : (uint16_t)0;
-      const int gn = n0 + r;
-      Bs[r][c] = (gn < N && gk < K) ? B[(int64_t)gk * sbk + (int64_t)gn * sbn] : (uint16_t)0;
+    const bool more = k0 + 32 < K;
+    if (more) {
+      ra = tile_load<AK>(A, sam, sak, M, K, m0, k0 + 32, t, va);
+      rb = tile_load<BK>(B, sbn, sbk, N, K, n0, k0 + 32, t, vb);
     }
+    uint4 a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const uint4*>(&As[wm + 16 * i + r][kq]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const uint4*>(&Bs[wn + 16 * j + r][kq]);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = mfma_k32(a[i], b[j], acc[i][j]);
     __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < 32; kk += 16) {
-      short4_t a[2], b[2];
-      const int kq = kk + 4 * (lane >> 4);
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int r = wm + 16 * i + (lane & 15);
-        a[i] = short4_t{(short)As[r][kq], (short)As[r][kq + 1], (short)As[r][kq + 2],
-                        (short)As[r][kq + 3]};
-      }
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int c = wn + 16 * j + (lane & 15);
-        b[j] = short4_t{(short)Bs[c][kq], (short)Bs[c][kq + 1], (short)Bs[c][kq + 2],
-                        (short)Bs[c][kq + 3]};
-      }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (more) {
+      tile_store<AK>(As, t, ra);
+      tile_store<BK>(Bs, t, rb);
+      __syncthreads();
     }
-    __syncthreads();
   }
   // D layout: lane holds D[4*(lane/16) + v][lane%16]
 #pragma unroll
@@ -125,18 +177,6 @@ __global__ __launch_bounds__(256) void gemm_bf16(const uint16_t* __restrict__ A,
         }
       }
     }
-}
-
-// ---- inference (rte_mlp_predict), v_mfma_f32_16x16x32_bf16 on gfx950 ----
-// Operand layout of the 16x16x32 MFMA used here: lane l supplies row (A) or
-// column (B) l % 16 and the 8 consecutive k values 8 * (l / 16) + 0..7 of a
-// 32-wide k step (A and B use the same k assignment, so only the order of
-// the k sum is implied); lane l holds D[4 * (l / 16) + v][l % 16], v = 0..3.
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ floatx4 mfma_k32(uint4 a, uint4 b, floatx4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
-                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
 __device__ __forceinline__ uint32_t pack2(float lo, float hi) {
@@ -734,19 +774,36 @@ hipError_t reserve_batch(Mlp* m, int n) {
 
 dim3 gemm_grid(int M, int N) { return dim3((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64)); }
 
+// gemm_bf16 with the operand layouts picked from the strides (k contiguous
+// when its stride is 1)
+template <int EPI>
+void launch_gemm(dim3 grid, hipStream_t st, const uint16_t* A, int64_t sam, int64_t sak,
+                 const uint16_t* B, int64_t sbk, int64_t sbn, int M, int N, int K,
+                 const float* bias, const float* act, float* out, uint16_t* out16) {
+  const bool ak = sak == 1, bk = sbk == 1;
+#define RTE_GEMM(AK, BK)                                                                    \
+  hipLaunchKernelGGL((rte::gemm_bf16<EPI, AK, BK>), grid, dim3(256), 0, st, A, sam, sak, B, \
+                     sbk, sbn, M, N, K, bias, act, out, out16)
+  if (ak && bk) RTE_GEMM(true, true);
+  else if (ak) RTE_GEMM(true, false);
+  else if (bk) RTE_GEMM(false, true);
+  else RTE_GEMM(false, false);
+#undef RTE_GEMM
+}
+
 // forward of n rows already converted to bf16 in X16; logits in Z
 hipError_t forward(Mlp* m, int n, hipStream_t st) {
   const int d0 = m->dims[0], d1 = m->dims[1], d2 = m->dims[2], d3 = m->dims[3];
   // H1 = relu(X W1 + b1): A = X16 (n x d0), B = W1 (d0 x d1, row-major)
-  hipLaunchKernelGGL(rte::gemm_bf16<rte::EPI_RELU>, gemm_grid(n, d1), dim3(256), 0, st,
+  launch_gemm<rte::EPI_RELU>(gemm_grid(n, d1), st,
                      m->X16.as<uint16_t>(), (int64_t)d0, (int64_t)1, m->W16[0].as<uint16_t>(),
                      (int64_t)d1, (int64_t)1, n, d1, d0, m->b[0].as<float>(), nullptr,
                      m->H1.as<float>(), m->H1b.as<uint16_t>());
-  hipLaunchKernelGGL(rte::gemm_bf16<rte::EPI_RELU>, gemm_grid(n, d2), dim3(256), 0, st,
+  launch_gemm<rte::EPI_RELU>(gemm_grid(n, d2), st,
                      m->H1b.as<uint16_t>(), (int64_t)d1, (int64_t)1, m->W16[1].as<uint16_t>(),
                      (int64_t)d2, (int64_t)1, n, d2, d1, m->b[1].as<float>(), nullptr,
                      m->H2.as<float>(), m->H2b.as<uint16_t>());
-  hipLaunchKernelGGL(rte::gemm_bf16<rte::EPI_LINEAR>, gemm_grid(n, d3), dim3(256), 0, st,
+  launch_gemm<rte::EPI_LINEAR>(gemm_grid(n, d3), st,
                      m->H2b.as<uint16_t>(), (int64_t)d2, (int64_t)1, m->W16[2].as<uint16_t>(),
                      (int64_t)d3, (int64_t)1, n, d3, d2, m->b[2].as<float>(), nullptr,
                      m->Z.as<float>(), nullptr);
@@ -759,28 +816,28 @@ hipError_t backward(Mlp* m, const int32_t* y, int n, hipStream_t st) {
   hipLaunchKernelGGL(rte::softmax_xent_kernel, dim3(n), dim3(256), 0, st, m->Z.as<float>(), d3, y,
                      n, m->dZ.as<float>(), m->dZb.as<uint16_t>(), m->loss.as<float>());
   // gW3 = H2^T dZ: (d2 x n) . (n x d3); A(m=i,k=r) = H2b[r*d2+i]
-  hipLaunchKernelGGL(rte::gemm_bf16<rte::EPI_STORE>, gemm_grid(d2, d3), dim3(256), 0, st,
+  launch_gemm<rte::EPI_STORE>(gemm_grid(d2, d3), st,
                      m->H2b.as<uint16_t>(), (int64_t)1, (int64_t)d2, m->dZb.as<uint16_t>(),
                      (int64_t)d3, (int64_t)1, d2, d3, n, nullptr, nullptr, m->gW[2].as<float>(),
                      nullptr);
   hipLaunchKernelGGL(rte::colsum_kernel, dim3((d3 + 255) / 256), dim3(256), 0, st,
                      m->dZ.as<float>(), n, d3, m->gb[2].as<float>());
   // dH2 = (dZ W3^T) * (H2 > 0): B(k=j, n=i) = W3[i*d3 + j]
-  hipLaunchKernelGGL(rte::gemm_bf16<rte::EPI_RELU_BWD>, gemm_grid(n, d2), dim3(256), 0, st,
+  launch_gemm<rte::EPI_RELU_BWD>(gemm_grid(n, d2), st,
                      m->dZb.as<uint16_t>(), (int64_t)d3, (int64_t)1, m->W16[2].as<uint16_t>(),
                      (int64_t)1, (int64_t)d3, n, d2, d3, nullptr, m->H2.as<float>(),
                      m->dH2.as<float>(), m->dH2b.as<uint16_t>());
-  hipLaunchKernelGGL(rte::gemm_bf16<rte::EPI_STORE>, gemm_grid(d1, d2), dim3(256), 0, st,
+  launch_gemm<rte::EPI_STORE>(gemm_grid(d1, d2), st,
                      m->H1b.as<uint16_t>(), (int64_t)1, (int64_t)d1, m->dH2b.as<uint16_t>(),
                      (int64_t)d2, (int64_t)1, d1, d2, n, nullptr, nullptr, m->gW[1].as<float>(),
                      nullptr);
   hipLaunchKernelGGL(rte::colsum_kernel, dim3((d2 + 255) / 256), dim3(256), 0, st,
                      m->dH2.as<float>(), n, d2, m->gb[1].as<float>());
-  hipLaunchKernelGGL(rte::gemm_bf16<rte::EPI_RELU_BWD>, gemm_grid(n, d1), dim3(256), 0, st,
+  launch_gemm<rte::EPI_RELU_BWD>(gemm_grid(n, d1), st,
                      m->dH2b.as<uint16_t>(), (int64_t)d2, (int64_t)1, m->W16[1].as<uint16_t>(),
                      (int64_t)1, (int64_t)d2, n, d1, d2, nullptr, m->H1.as<float>(),
                      m->dH1.as<float>(), m->dH1b.as<uint16_t>());
-  hipLaunchKernelGGL(rte::gemm_bf16<rte::EPI_STORE>, gemm_grid(d0, d1), dim3(256), 0, st,
+  launch_gemm<rte::EPI_STORE>(gemm_grid(d0, d1), st,
                      m->X16.as<uint16_t>(), (int64_t)1, (int64_t)d0, m->dH1b.as<uint16_t>(),
                      (int64_t)d1, (int64_t)1, d0, d1, n, nullptr, nullptr, m->gW[0].as<float>(),
                      nullptr);
