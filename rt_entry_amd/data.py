@@ -292,16 +292,22 @@ def _splitmix64(z: np.ndarray) -> np.ndarray:
 
 
 def add_entry_links(g: np.ndarray, pids: np.ndarray, r: int, window: int = 0, seed: int = 1,
-                    tries: int = 8) -> np.ndarray:
+                    tries: int = 8, rows=None) -> np.ndarray:
     """Numpy definition of rte_graph_add_entry_links: the last r slots of
     row u become points of the leaves within `window` ids of u's leaf (in
     the pid table's order), candidate t of slot j drawn as
     splitmix64(seed + ((u << 16) | (j << 8) | t) * 0x9E3779B97F4A7C15) mod
     the window's point count, redrawn when equal to u, a kept entry or an
-    earlier link, padding (n) after `tries` draws."""
+    earlier link, padding (n) after `tries` draws.  A row's links depend on
+    that row alone, so `rows` (node ids) returns just those rows, in order."""
     n, degree = g.shape
     n_leaf, cap = pids.shape
-    out = np.array(g, np.int32, copy=True)
+    if rows is not None:
+        rows = np.asarray(rows, np.int64)
+        out = np.array(g[rows], np.int32, copy=True)
+    else:
+        rows = np.arange(n, dtype=np.int64)
+        out = np.array(g, np.int32, copy=True)
     if r == 0:
         return out
     leaf_of = np.full(n, -1, np.int64)
@@ -311,10 +317,10 @@ def add_entry_links(g: np.ndarray, pids: np.ndarray, r: int, window: int = 0, se
     keep = degree - r
     golden = np.uint64(0x9E3779B97F4A7C15)
     with np.errstate(over="ignore"):
-        for u in range(n):
+        for o, u in enumerate(rows.tolist()):
             l = leaf_of[u]
             if l < 0:
-                out[u, keep:] = n
+                out[o, keep:] = n
                 continue
             lo, hi = max(0, l - window), min(n_leaf - 1, l + window)
             total = size[lo:hi + 1].sum()
@@ -329,11 +335,11 @@ def add_entry_links(g: np.ndarray, pids: np.ndarray, r: int, window: int = 0, se
                         idx -= int(size[x])
                         x += 1
                     c = int(pids[x, idx])
-                    if c != u and c not in out[u, :keep] and c not in links:
+                    if c != u and c not in out[o, :keep] and c not in links:
                         pick = c
                         break
                 links.append(pick)
-            out[u, keep:] = links
+            out[o, keep:] = links
     return out
 
 

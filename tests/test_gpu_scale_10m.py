@@ -198,3 +198,35 @@ def test_predict_entry(ctx, big):
                     f"confidences min_conf {min_conf}")
         if min_conf > 1:
             assert_same(got.cpu().numpy(), f.hits_h, "fallback = kd-tree leaves")
+
+
+def test_entry_links_and_lds_queue_beam512(ctx, big):
+    """The combined workload's graph form at full size: entry-space links
+    (16 per row, the row's own leaf) added on the device equal the numpy
+    definition on 2048 sampled rows; then a uniform batched search at beam
+    512 with the beam in an LDS queue (the launch the autotuner picks on
+    this workload; row byte offsets pass 2^32) equals the oracle on 128
+    sampled queries of the linked graph."""
+    from rt_entry_amd import data as D
+    from rt_entry_amd import make_params
+    f = big
+    _setup(ctx, f)
+    g = f.graph.clone()
+    ctx.graph_add_entry_links(g, 16, window=0, seed=1)
+    ctx.synchronize()
+    g_h = g.cpu().numpy()
+    rows = np.sort(np.random.default_rng(23).choice(N, 2048, replace=False))
+    ref = D.add_entry_links(f.graph_h, f.pids, 16, window=0, seed=1, rows=rows)
+    assert_same(g_h[rows], ref, "entry links (sampled rows)")
+    ctx.set_graph(g)
+    p = make_params(k=10, beam=512, max_iter=40, mode=0, queue=1, order=1)
+    ids, dists = ctx.search_batched(f.pq, f.hits, None, None, p, k_stride=10)
+    ctx.synchronize()
+    ids, dists = ids.cpu().numpy(), dists.cpu().numpy()
+    _check_rows(f, ids, dists, 10)
+    sample = np.random.default_rng(24).choice(NQ, 128, replace=False)
+    po = O.make_params(k=10, beam=512, max_iter=40, mode=0)
+    io, do = O.search(f.pb_h, g_h, f.pq_h[sample], f.pids, f.hits_h[sample], po)
+    assert_same(ids[sample], io, "ids vs oracle")
+    assert_same(dists[sample].view(np.int32), do.view(np.int32), "dists vs oracle")
+    ctx.set_graph(f.graph)
