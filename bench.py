@@ -610,8 +610,8 @@ def main():
 
     def entries(use):
         """Entry leaves of the projected queries: kd-tree, or the network
-        with kd-tree fallback (one call; the kd-tree walk overlaps the
-        network on a second stream)."""
+        with kd-tree fallback (one call; the kd-tree walk runs beside the
+        network's hidden layers, in the same launch)."""
         if use:
             return ctx.predict_entry(pq, min_conf=args.min_conf, out=learned)
         return ctx.select_entry(pq, out=hits)

@@ -322,9 +322,8 @@ int rte_mlp_predict(rte_ctx* ctx, const float* X, int64_t n, const int32_t* fall
  * max(3, d_in)): the network's leaf, or the kd-tree leaf of rte_select_entry
  * when the softmax maximum is below min_conf.  Same results as
  * rte_select_entry followed by rte_mlp_predict; the kd-tree traversal runs
- * on a second stream of the context, overlapping the network, and is
- * joined before the fallback is applied.  Needs rte_set_leaves and
- * rte_mlp_create. */
+ * in workgroups of the hidden layers' launch, beside the network, on the
+ * context's stream.  Needs rte_set_leaves and rte_mlp_create. */
 int rte_predict_entry(rte_ctx* ctx, const float* PQ, int64_t nq, int stride, float min_conf,
                       int32_t* leaf_out, float* conf_out);
 /* Input normalisation x' = (x - mean) * inv_std per feature (host arrays of

@@ -481,8 +481,8 @@ class Context:
 
     def predict_entry(self, PQ, min_conf: float = 0.5, out=None, conf=None):
         """Entry leaves of device PCA queries: the network's leaf, or the
-        kd-tree leaf below min_conf (the kd-tree selection overlaps the
-        network on a second stream)."""
+        kd-tree leaf below min_conf (the kd-tree selection runs beside the
+        network's hidden layers, in the same launch)."""
         import torch
         nq, qs = PQ.shape
         if out is None:

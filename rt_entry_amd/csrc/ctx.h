@@ -91,10 +91,8 @@ struct rte_ctx {
   std::vector<unsigned char> gkey;
 
   rte::Mlp* mlp = nullptr;
-  // rte_predict_entry: kd-tree leaves computed on a side stream (default
-  // priority) while the network runs, joined before the fallback is applied
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // rte_predict_entry: kd-tree leaves (the fallback), selected in the hidden
+  // layers' launch
   DevBuf kd_hits;
   ~rte_ctx();
 };

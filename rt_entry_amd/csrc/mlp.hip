@@ -18,6 +18,7 @@
 
 #include "ctx.h"
 #include "entry.h"
+#include "entry_impl.h"
 #include "mlp.h"
 
 namespace rte {
@@ -254,20 +255,43 @@ __device__ __forceinline__ void hidden_layer(const uint16_t* __restrict__ As, in
   }
 }
 
+struct HiddenArgs {
+  const float* X;  // input rows, ldx floats apart
+  int64_t ldx;
+  int64_t n;
+  int d0;
+  const float* mean;
+  const float* inv;
+  const uint16_t* W1T;
+  const float* b1;
+  int d1;
+  const uint16_t* W2T;
+  const float* b2;
+  int d2;
+  uint16_t* H2;  // n x d2 bf16
+};
+
+// rows 32 blk .. 32 blk + 31 of the hidden layers (the workgroup's 256
+// threads; lds: dynamic LDS of the size stated above)
 template <int CB>
-__global__ __launch_bounds__(256) void mlp_hidden_kernel(
-    const float* __restrict__ X, int64_t ldx, int64_t n, int d0, const float* __restrict__ mean,
-    const float* __restrict__ inv, const uint16_t* __restrict__ W1T,
-    const float* __restrict__ b1, int d1, const uint16_t* __restrict__ W2T,
-    const float* __restrict__ b2, int d2, uint16_t* __restrict__ H2) {
-  extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
+__device__ __forceinline__ void hidden_rows(const HiddenArgs& a, int64_t blk, uint16_t* lds) {
+  const float* __restrict__ X = a.X;
+  const int64_t ldx = a.ldx, n = a.n;
+  const int d0 = a.d0, d1 = a.d1, d2 = a.d2;
+  const float* __restrict__ mean = a.mean;
+  const float* __restrict__ inv = a.inv;
+  const uint16_t* __restrict__ W1T = a.W1T;
+  const uint16_t* __restrict__ W2T = a.W2T;
+  const float* __restrict__ b1 = a.b1;
+  const float* __restrict__ b2 = a.b2;
+  uint16_t* __restrict__ H2 = a.H2;
   const int ld0 = d0 + 8, ld1 = d1 + 8, ld2 = d2 + 8;
   uint16_t* Xs = lds;
   uint16_t* H1s = lds + 32 * ld0;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int r = lane & 15, g = lane >> 4;
-  const int64_t m0 = (int64_t)blockIdx.x * 32;
+  const int64_t m0 = blk * 32;
   WFrag<CB> w;  // first W1 group, in flight while the input tile is built
   load_wfrag<CB>(w, W1T, d1, d0, 0, wave, r, g);
   // biases of this lane's columns, loaded once with the first group (columns
@@ -333,6 +357,38 @@ __global__ __launch_bounds__(256) void mlp_hidden_kernel(
       *reinterpret_cast<uint4*>(H2 + (m0 + row) * d2 + 8 * ch) =
           *reinterpret_cast<const uint4*>(&Hs[row * ld2 + 8 * ch]);
   }
+}
+
+template <int CB>
+__global__ __launch_bounds__(256) void mlp_hidden_kernel(HiddenArgs a) {
+  extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
+  hidden_rows<CB>(a, blockIdx.x, lds);
+}
+
+struct SelectArgs {
+  const BoxNode* nodes;
+  int n_nodes;
+  const float* pq;  // PCA queries, stride floats apart
+  int64_t nq;
+  int stride;
+  int32_t* hits;
+};
+
+// The hidden layers and the kd-tree entry selection of the same queries in
+// one launch: workgroups below sel_blocks select the entries of 256
+// queries each (the walk of select_entry_kernel), the others run the hidden
+// layers of 32 rows each.  The selection (the fallback leaves) so runs
+// beside the network with no second stream and no event between them.
+// Dynamic LDS: the larger of the hidden layers' and kTopNodes * 32 bytes.
+template <int CB>
+__global__ __launch_bounds__(256) void mlp_hidden_select_kernel(HiddenArgs a, SelectArgs sa,
+                                                                int sel_blocks) {
+  extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
+  if ((int)blockIdx.x < sel_blocks)
+    select_entry_block(sa.nodes, sa.n_nodes, reinterpret_cast<BoxNode*>(lds), sa.pq, sa.nq,
+                       sa.stride, sa.hits, (int64_t)blockIdx.x * 256);
+  else
+    hidden_rows<CB>(a, (int64_t)blockIdx.x - sel_blocks, lds);
 }
 
 // Softmax partial of one row over a set of columns: maximum logit, its lowest
@@ -406,45 +462,114 @@ __device__ __forceinline__ void head_dma_tile(const uint16_t* WT, const float* b
                                      4, 0, 0);
 }
 
-// Logits of one tile for the wave's 32 rows, fed into the online softmax:
-// the accumulators start from the bias; B fragments of k step s + 1 are read
-// while step s multiplies.
-template <int KS>
-__device__ __forceinline__ void head_tile(const uint4* Bt, const float* bt, int r, int g, int xr,
-                                          const uint4 (&A)[2][KS], int n0, float (&mx)[2][4],
-                                          float (&se)[2][4], int (&ix)[2][4]) {
+// Logits of one 64-column tile for the wave's 32 rows: acc = bias + H W^T,
+// the accumulators starting from the bias; the B fragments of k step s + PD
+// are read while step s multiplies.
+template <int KS, int PD>
+__device__ __forceinline__ void head_logits(const uint4* Bt, const float* bt, int r, int g, int xr,
+                                            const uint4 (&A)[2][KS], floatx4 (&acc)[2][4]) {
   constexpr int CPR = 4 * KS;
-  floatx4 acc[2][4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const float bj = bt[16 * j + r];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) acc[rb][j] = floatx4{bj, bj, bj, bj};
   }
-  uint4 bc[4], bn[4];
+  uint4 b[PD + 1][4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) bc[j] = Bt[(16 * j + r) * CPR + (g ^ xr)];
+  for (int p = 0; p < PD && p < KS; ++p)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[p][j] = Bt[(16 * j + r) * CPR + ((4 * p + g) ^ xr)];
 #pragma unroll
   for (int s = 0; s < KS; ++s) {
-    if (s + 1 < KS) {
+    if (s + PD < KS) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bn[j] = Bt[(16 * j + r) * CPR + ((4 * (s + 1) + g) ^ xr)];
+      for (int j = 0; j < 4; ++j)
+        b[(s + PD) % (PD + 1)][j] = Bt[(16 * j + r) * CPR + ((4 * (s + PD) + g) ^ xr)];
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) acc[rb][j] = mfma_k32(A[rb][s], bc[j], acc[rb][j]);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) bc[j] = bn[j];
+      for (int rb = 0; rb < 2; ++rb)
+        acc[rb][j] = mfma_k32(A[rb][s], b[s % (PD + 1)][j], acc[rb][j]);
   }
-  // columns >= N carry a -inf bias (zero weights), so their logits are -inf
-  // and drop out of max and sum
+}
+
+// One tile's logits into the online softmax of the wave's 32 rows.  Columns
+// >= N carry a -inf bias (zero weights), so their logits are -inf and drop
+// out of max and sum.
+__device__ __forceinline__ void head_softmax(const floatx4 (&acc)[2][4], int col0,
+                                             float (&mx)[2][4], float (&se)[2][4],
+                                             int (&ix)[2][4]) {
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
     for (int v = 0; v < 4; ++v)
       soft_update(mx[rb][v], se[rb][v], ix[rb][v], acc[rb][0][v], acc[rb][1][v], acc[rb][2][v],
-                  acc[rb][3][v], n0 + r);
+                  acc[rb][3][v], col0);
+}
+
+template <int KS>
+__device__ __forceinline__ void head_tile(const uint4* Bt, const float* bt, int r, int g, int xr,
+                                          const uint4 (&A)[2][KS], int n0, float (&mx)[2][4],
+                                          float (&se)[2][4], int (&ix)[2][4]) {
+  floatx4 acc[2][4];
+  head_logits<KS, 1>(Bt, bt, r, g, xr, A, acc);
+  head_softmax(acc, n0 + r, mx, se, ix);
+}
+
+// bf16 H2 fragments of the wave's 32 rows (from row m0), zero past n
+template <int KS>
+__device__ __forceinline__ void head_rows(const uint16_t* __restrict__ H, int64_t n, int64_t m0,
+                                          int r, int g, uint4 (&A)[2][KS]) {
+  constexpr int K = 32 * KS;
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb) {
+    const int64_t row = m0 + 16 * rb + r;
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+      A[rb][s] = row < n ? *reinterpret_cast<const uint4*>(H + row * K + 32 * s + 8 * g)
+                         : make_uint4(0, 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ void head_state_init(float (&mx)[2][4], float (&se)[2][4],
+                                                int (&ix)[2][4]) {
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      mx[rb][v] = -1e30f;  // below every logit, finite (see soft_update)
+      se[rb][v] = 0.f;
+      ix[rb][v] = 0x7fffffff;
+    }
+}
+
+// merge the 16 lanes of each row (lanes differing in bits 0..3) and write
+// one SoftPart per (row, column group)
+__device__ __forceinline__ void head_store(float (&mx)[2][4], float (&se)[2][4], int (&ix)[2][4],
+                                           int64_t n, int64_t m0, int r, int g, int G, int grp,
+                                           SoftPart* __restrict__ parts) {
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        const float om = __shfl_xor(mx[rb][v], o, 64);
+        const float os = __shfl_xor(se[rb][v], o, 64);
+        const int oi = __shfl_xor(ix[rb][v], o, 64);
+        soft_merge(mx[rb][v], se[rb][v], ix[rb][v], om, os, oi);
+      }
+      const int64_t row = m0 + 16 * rb + 4 * g + v;
+      if (r == 0 && row < n) {
+        SoftPart p;
+        p.mx = mx[rb][v];
+        p.se = se[rb][v];
+        p.ix = ix[rb][v];
+        parts[row * G + grp] = p;
+      }
+    }
 }
 
 // Output layer fused with the softmax reduction, for 128 rows x one group of
@@ -492,24 +617,10 @@ __global__ __launch_bounds__(256, 2) void mlp_head_kernel(const uint16_t* __rest
   const int ntile = (c1 - c0 + 63) >> 6;
   head_dma_tile<KS>(WT, bias, c0, soff, Bt0, bias0, wave, lane);
   uint4 A[2][KS];
-#pragma unroll
-  for (int rb = 0; rb < 2; ++rb) {
-    const int64_t row = m0 + 16 * rb + r;
-#pragma unroll
-    for (int s = 0; s < KS; ++s)
-      A[rb][s] = row < n ? *reinterpret_cast<const uint4*>(H + row * K + 32 * s + 8 * g)
-                         : make_uint4(0, 0, 0, 0);
-  }
+  head_rows<KS>(H, n, m0, r, g, A);
   float mx[2][4], se[2][4];
   int ix[2][4];
-#pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      mx[rb][v] = -1e30f;  // below every logit, finite (see soft_update)
-      se[rb][v] = 0.f;
-      ix[rb][v] = 0x7fffffff;
-    }
+  head_state_init(mx, se, ix);
   __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): the first tile is in LDS
   __syncthreads();
   // two tiles per trip, so each half names its buffers statically; a tile
@@ -530,27 +641,117 @@ __global__ __launch_bounds__(256, 2) void mlp_head_kernel(const uint16_t* __rest
       __syncthreads();
     }
   }
-  // merge the 16 lanes of each row (lanes differing in bits 0..3)
+  head_store(mx, se, ix, n, m0, r, g, G, grp, parts);
+}
+
+// Copy of one 64-column tile of W3^T and its 64 biases into one LDS buffer
+// of the eight-wave kernel: the tile is 64 * CPR 16-byte units, copy i of
+// wave w moves units 64 * (8 i + w) + lane (KS / 2 copies per wave), the
+// biases follow the tile (wave 0).
+template <int KS>
+__device__ __forceinline__ void head8_dma_tile(const uint16_t* WT, const float* bias, int col0,
+                                               const int (&soff)[KS / 2], uint4* dst, int wave,
+                                               int lane) {
+  constexpr int K = 32 * KS;
+  constexpr int CPR = 4 * KS;
+#pragma unroll
+  for (int i = 0; i < KS / 2; ++i)
+    __builtin_amdgcn_global_load_lds((global_void_t*)(WT + (int64_t)col0 * K + soff[i]),
+                                     (lds_void_t*)(dst + 64 * (8 * i + wave)), 16, 0, 0);
+  if (wave == 0)
+    __builtin_amdgcn_global_load_lds((global_void_t*)(bias + col0 + lane),
+                                     (lds_void_t*)(dst + 64 * CPR), 4, 0, 0);
+}
+
+// The output layer for 256 rows x one group of `cg` columns per workgroup of
+// eight waves (KS = 2, 4, 8).  On this part a SIMD does not run one wave's
+// MFMAs beside another wave's softmax arithmetic by itself when both waves
+// reach them at once, so the work is skewed: iteration t computes the
+// logits of tile t into one accumulator set and the softmax update of tile
+// t - 1 from the other; waves 0-3 issue the MFMAs first and waves 4-7 the
+// softmax first (a scheduling barrier keeps each order), and the two waves
+// that share a SIMD (w and w + 4) keep its matrix unit and its VALU busy
+// together.  Each 64-column tile (and its biases) is copied straight into
+// one of two LDS buffers, the next tile's copy in flight during the
+// current iteration.  Per (row, group) the result equals mlp_head_kernel's:
+// the same MFMA order and the same softmax update sequence per lane.
+// Static LDS 2 * (64 * K * 2 + 256) bytes; one workgroup per CU.
+template <int KS>
+__global__ __launch_bounds__(512, 1) void mlp_head8_kernel(const uint16_t* __restrict__ H,
+                                                           int64_t n,
+                                                           const uint16_t* __restrict__ WT,
+                                                           const float* __restrict__ bias, int N,
+                                                           int cg, SoftPart* __restrict__ parts) {
+  static_assert(KS == 2 || KS == 4 || KS == 8, "eight-wave head: KS in {2, 4, 8}");
+  constexpr int K = 32 * KS;
+  constexpr int CPR = K / 8;
+  constexpr int SWM = (CPR < 16 ? CPR : 16) - 1;
+  __shared__ uint4 B0[64 * CPR + 16];  // tile, then its 64 biases
+  __shared__ uint4 B1[64 * CPR + 16];
+  const float* b0 = reinterpret_cast<const float*>(B0 + 64 * CPR);
+  const float* b1 = reinterpret_cast<const float*>(B1 + 64 * CPR);
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int xr = r & SWM;
+  const int grp = blockIdx.x, G = gridDim.x;
+  const int c0 = grp * cg;
+  const int c1 = min(N, c0 + cg);
+  const int64_t m0 = (int64_t)blockIdx.y * 256 + 32 * wave;
+  const bool mfma_first = wave < 4;
+  int soff[KS / 2];
+#pragma unroll
+  for (int i = 0; i < KS / 2; ++i) {
+    const int u = 64 * (8 * i + wave) + lane;
+    const int c = u / CPR;
+    soff[i] = c * K + 8 * ((u % CPR) ^ (c & SWM));
+  }
+  const int ntile = (c1 - c0 + 63) >> 6;
+  head8_dma_tile<KS>(WT, bias, c0, soff, B0, wave, lane);
+  uint4 A[2][KS];
+  head_rows<KS>(H, n, m0, r, g, A);
+  float mx[2][4], se[2][4];
+  int ix[2][4];
+  head_state_init(mx, se, ix);
+  // acc1 starts as -inf logits: the softmax update of iteration 0 leaves the
+  // state unchanged
+  floatx4 acc0[2][4], acc1[2][4];
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-#pragma unroll
-      for (int o = 1; o < 16; o <<= 1) {
-        const float om = __shfl_xor(mx[rb][v], o, 64);
-        const float os = __shfl_xor(se[rb][v], o, 64);
-        const int oi = __shfl_xor(ix[rb][v], o, 64);
-        soft_merge(mx[rb][v], se[rb][v], ix[rb][v], om, os, oi);
-      }
-      const int64_t row = m0 + 16 * rb + 4 * g + v;
-      if (r == 0 && row < n) {
-        SoftPart p;
-        p.mx = mx[rb][v];
-        p.se = se[rb][v];
-        p.ix = ix[rb][v];
-        parts[row * G + grp] = p;
-      }
-    }
+    for (int j = 0; j < 4; ++j) acc1[rb][j] = floatx4{-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): rows and the first tile
+  __syncthreads();
+  // two iterations per trip, so buffers and accumulator sets are named
+  // statically; iteration tl reads buffer tl & 1 and copies tile tl + 1
+  // into the other one (free: every wave passed the barrier after tl - 1)
+#define RTE_HEAD8_ITER(BUF, BB, NBUF, ACC, PACC, TL)                                       \
+  {                                                                                        \
+    const int tl_ = (TL);                                                                  \
+    if (tl_ + 1 < ntile) head8_dma_tile<KS>(WT, bias, c0 + 64 * (tl_ + 1), soff, NBUF, wave, lane); \
+    if (mfma_first) {                                                                      \
+      head_logits<KS, 2>(BUF, BB, r, g, xr, A, ACC);                                       \
+      __builtin_amdgcn_sched_barrier(0);                                                   \
+      head_softmax(PACC, c0 + 64 * (tl_ - 1) + r, mx, se, ix);                             \
+    } else {                                                                               \
+      head_softmax(PACC, c0 + 64 * (tl_ - 1) + r, mx, se, ix);                             \
+      __builtin_amdgcn_sched_barrier(0);                                                   \
+      head_logits<KS, 2>(BUF, BB, r, g, xr, A, ACC);                                       \
+    }                                                                                      \
+    __builtin_amdgcn_s_waitcnt(0x0f70);                                                    \
+    __syncthreads();                                                                       \
+  }
+  for (int tl = 0; tl < ntile; tl += 2) {
+    RTE_HEAD8_ITER(B0, b0, B1, acc0, acc1, tl)
+    if (tl + 1 < ntile) RTE_HEAD8_ITER(B1, b1, B0, acc1, acc0, tl + 1)
+  }
+#undef RTE_HEAD8_ITER
+  // the last tile's softmax update
+  if ((ntile - 1) & 1)
+    head_softmax(acc1, c0 + 64 * (ntile - 1) + r, mx, se, ix);
+  else
+    head_softmax(acc0, c0 + 64 * (ntile - 1) + r, mx, se, ix);
+  head_store(mx, se, ix, n, m0, r, g, G, grp, parts);
 }
 
 // row-wise combination of the softmax partials: leaf = lowest column of the
@@ -737,9 +938,6 @@ void mlp_destroy(Mlp* m) { delete m; }
 rte_ctx::~rte_ctx() {
   rte::mlp_destroy(mlp);
   if (gexec) (void)hipGraphExecDestroy(gexec);
-  if (ev_fork) (void)hipEventDestroy(ev_fork);
-  if (ev_join) (void)hipEventDestroy(ev_join);
-  if (side) (void)hipStreamDestroy(side);
 }
 
 namespace {
@@ -1070,7 +1268,7 @@ namespace {
 
 // Prediction of n rows of X (row stride ldx floats).  The fallback leaves
 // are `fallback`, or with sel_pq the kd-tree leaves of sel_pq (row stride
-// sel_stride), selected on the side stream while the network runs.
+// sel_stride), selected by workgroups of the hidden layers' launch.
 int predict_impl(rte_ctx* c, const float* X, int64_t ldx, int64_t n, const int32_t* fallback,
                  const float* sel_pq, int sel_stride, float min_conf, int32_t* leaf_out,
                  float* conf_out) {
@@ -1140,20 +1338,34 @@ int predict_impl(rte_ctx* c, const float* X, int64_t ldx, int64_t n, const int32
                    : cb == 2 ? (const void*)rte::mlp_hidden_kernel<2>
                    : cb == 4 ? (const void*)rte::mlp_hidden_kernel<4>
                              : (const void*)rte::mlp_hidden_kernel<8>;
+  // output layer: the eight-wave kernel (256 rows per workgroup) for K = 64,
+  // 128 and 256; the four-wave kernel (128 rows) for K = 32 (one k step) and
+  // K = 512 (whose row fragments fill the registers)
+  const bool head8 = ks == 2 || ks == 4 || ks == 8;
+  const int head_threads = head8 ? 512 : 256, rows_per_wg = head8 ? 256 : 128;
   const void* ok = ks == 1    ? (const void*)rte::mlp_head_kernel<1>
-                   : ks == 2  ? (const void*)rte::mlp_head_kernel<2>
-                   : ks == 4  ? (const void*)rte::mlp_head_kernel<4>
-                   : ks == 8  ? (const void*)rte::mlp_head_kernel<8>
+                   : ks == 2  ? (const void*)rte::mlp_head8_kernel<2>
+                   : ks == 4  ? (const void*)rte::mlp_head8_kernel<4>
+                   : ks == 8  ? (const void*)rte::mlp_head8_kernel<8>
                               : (const void*)rte::mlp_head_kernel<16>;
   MCHK(c, hipFuncSetAttribute(ok, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_o),
        "head kernel LDS size");
   MCHK(c, hipFuncSetAttribute(hk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_h),
        "hidden kernel LDS size");
+  const size_t lds_hs = std::max(lds_h, (size_t)rte::kTopNodes * sizeof(rte::BoxNode));
+  if (sel_pq) {
+    const void* hsk = cb == 1   ? (const void*)rte::mlp_hidden_select_kernel<1>
+                      : cb == 2 ? (const void*)rte::mlp_hidden_select_kernel<2>
+                      : cb == 4 ? (const void*)rte::mlp_hidden_select_kernel<4>
+                                : (const void*)rte::mlp_hidden_select_kernel<8>;
+    MCHK(c, hipFuncSetAttribute(hsk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_hs),
+         "hidden + selection kernel LDS size");
+  }
   if (m->head_slots == 0) {
     int cus = 0, per_cu = 0;
     MCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device),
          "CU count");
-    MCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ok, 256, lds_o),
+    MCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ok, head_threads, lds_o),
          "head kernel occupancy");
     m->head_slots = std::max(1, cus) * std::max(1, per_cu);
   }
@@ -1162,26 +1374,14 @@ int predict_impl(rte_ctx* c, const float* X, int64_t ldx, int64_t n, const int32
   // round of workgroups on some CUs would nearly double the kernel's time),
   // each group a whole number of 64-column tiles.
   const int ntile = (d3 + 63) / 64;
-  const int64_t rblk = (b0 + 127) / 128;
+  const int64_t rblk = (b0 + rows_per_wg - 1) / rows_per_wg;
   const int gw = (int)std::max<int64_t>(1, std::min<int64_t>(ntile, m->head_slots / rblk));
   const int cg = (ntile + gw - 1) / gw * 64;
   const int G = (d3 + cg - 1) / cg;
   MCHK(c, m->parts.reserve((size_t)b0 * G * sizeof(rte::SoftPart)), "softmax partials");
-  if (sel_pq) {  // kd-tree leaves on the side stream, overlapping the network
-    if (!c->side) {
-      MCHK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking), "side stream");
-      MCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "event");
-      MCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "event");
-    }
-    MCHK(c, hipEventRecord(c->ev_fork, c->stream), "fork");
-    MCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0), "fork");
-    MCHK(c, rte::launch_select_entry(c->nodes.as<rte::BoxNode>(), c->n_nodes, sel_pq, n,
-                                     sel_stride, c->kd_hits.as<int32_t>(), c->side),
-         "select entry");
-    MCHK(c, hipEventRecord(c->ev_join, c->side), "join");
-    fallback = c->kd_hits.as<int32_t>();
-  }
-  bool joined = sel_pq == nullptr;
+  // with sel_pq the kd-tree leaves (the fallback) are selected in
+  // workgroups of the hidden-layer launch
+  if (sel_pq) fallback = c->kd_hits.as<int32_t>();
   for (int64_t s = 0; s < n; s += chunk) {
     const int64_t b = std::min<int64_t>(chunk, n - s);
     const float* Xs = X + s * ldx;
@@ -1192,33 +1392,47 @@ int predict_impl(rte_ctx* c, const float* X, int64_t ldx, int64_t n, const int32
     const uint16_t* W3T = m->WT16[2].as<uint16_t>();
     uint16_t* H2 = m->P2.as<uint16_t>();
     rte::SoftPart* parts = m->parts.as<rte::SoftPart>();
-    const dim3 gh((unsigned)((b + 31) / 32));
-#define HID(CB)                                                                                   \
-  hipLaunchKernelGGL(rte::mlp_hidden_kernel<CB>, gh, dim3(256), lds_h, c->stream, Xs, ldx, b, d0, \
-                     mean, inv, W1T, m->b[0].as<float>(), d1, W2T, m->b[1].as<float>(), d2, H2)
-    switch (cb) {
-      case 1: HID(1); break;
-      case 2: HID(2); break;
-      case 4: HID(4); break;
-      default: HID(8); break;
-    }
+    const rte::HiddenArgs ha{Xs, ldx, b, d0, mean, inv, W1T, m->b[0].as<float>(), d1,
+                             W2T, m->b[1].as<float>(), d2, H2};
+    const unsigned nh = (unsigned)((b + 31) / 32);
+    if (sel_pq) {
+      const rte::SelectArgs sa{c->nodes.as<rte::BoxNode>(), c->n_nodes, sel_pq + s * sel_stride,
+                               b, sel_stride, c->kd_hits.as<int32_t>() + s};
+      const int sel_blocks = (int)((b + 255) / 256);
+      const dim3 gs(nh + (unsigned)sel_blocks);
+#define HIDSEL(CB)                                                                          \
+  hipLaunchKernelGGL(rte::mlp_hidden_select_kernel<CB>, gs, dim3(256), lds_hs, c->stream, ha, \
+                     sa, sel_blocks)
+      switch (cb) {
+        case 1: HIDSEL(1); break;
+        case 2: HIDSEL(2); break;
+        case 4: HIDSEL(4); break;
+        default: HIDSEL(8); break;
+      }
+#undef HIDSEL
+    } else {
+#define HID(CB) \
+  hipLaunchKernelGGL(rte::mlp_hidden_kernel<CB>, dim3(nh), dim3(256), lds_h, c->stream, ha)
+      switch (cb) {
+        case 1: HID(1); break;
+        case 2: HID(2); break;
+        case 4: HID(4); break;
+        default: HID(8); break;
+      }
 #undef HID
-    const dim3 go((unsigned)G, (unsigned)((b + 127) / 128));
-#define HEAD(KS)                                                                              \
-  hipLaunchKernelGGL(rte::mlp_head_kernel<KS>, go, dim3(256), lds_o, c->stream, H2, b, W3T,     \
+    }
+    const dim3 go((unsigned)G, (unsigned)((b + rows_per_wg - 1) / rows_per_wg));
+#define HEAD(KERNEL)                                                                            \
+  hipLaunchKernelGGL(KERNEL, go, dim3(head_threads), lds_o, c->stream, H2, b, W3T,               \
                      m->b3p.as<float>(), d3, cg, parts)
     switch (ks) {
-      case 1: HEAD(1); break;
-      case 2: HEAD(2); break;
-      case 4: HEAD(4); break;
-      case 8: HEAD(8); break;
-      default: HEAD(16); break;
+      case 1: HEAD(rte::mlp_head_kernel<1>); break;
+      case 2: HEAD(rte::mlp_head8_kernel<2>); break;
+      case 4: HEAD(rte::mlp_head8_kernel<4>); break;
+      case 8: HEAD(rte::mlp_head8_kernel<8>); break;
+      default: HEAD(rte::mlp_head_kernel<16>); break;
     }
 #undef HEAD
-    if (!joined) {
-      MCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0), "join");
-      joined = true;
-    }
     hipLaunchKernelGGL(rte::predict_combine, dim3((unsigned)((b + 255) / 256)), dim3(256), 0,
                        c->stream, parts, G, b, fallback ? fallback + s : nullptr, min_conf,
                        leaf_out + s, conf_out ? conf_out + s : nullptr);
