@@ -248,10 +248,12 @@ int rte_graph_add_entry_links(rte_ctx* ctx, int32_t* graph, int64_t np, int degr
  * generalises GraphSearchKernel's launch (graph.cu:320-323) to mixed k and
  * beam.  p->k is ignored.
  * With k_per_q = beam_per_q = NULL the batch is uniform (every query uses
- * p->k and p->beam, k_stride >= p->k): the captured graph is the search
- * kernel alone over p->grid workgroups (0: the resident maximum), its waves
- * taking queries from a work counter that the launch leaves at zero;
- * results equal rte_search's.  Batched calls on one context must not
+ * p->k and p->beam, k_stride >= p->k): the search kernel (after the leaf
+ * sort with RTE_ORDER_LEAF) is launched directly, not through a graph (on
+ * this stack a graph launch left the GPU idle for several microseconds on
+ * either side of it), over p->grid workgroups (0: the resident maximum),
+ * its waves taking queries from a work counter that the launch leaves at
+ * zero; results equal rte_search's.  Batched calls on one context must not
  * overlap (they share the counter). */
 int rte_search_batched(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_stride,
                        const int32_t* leaf, const int32_t* k_per_q,

@@ -217,8 +217,9 @@ class Context:
 
     def search_batched(self, PQ, leaf, k_per_q, beam_per_q, params: Params, k_stride=None,
                        out=None, beam_bound=None):
-        """Batched launch (captured hipGraph).  With per-query k_per_q and
-        beam_per_q the batch is uneven; beam_bound is its largest effective
+        """Batched launch (an uneven batch: a captured hipGraph of the batch
+        preparation and the search; a uniform batch: direct launches).  With
+        per-query k_per_q and beam_per_q the batch is uneven; beam_bound is its largest effective
         beam (selects the kernel; default: computed from the arrays).  With
         both None the batch is uniform: every query uses params.k and
         params.beam (results equal search())."""
