@@ -74,3 +74,44 @@ def test_ragged_batches_match_oracle(ctx, int_workload, nq):
             bi, bd = ctx.search_batched(Q, hits, None, None, make_params(k=k, beam=beam))
             assert_same(bi, io, "batched ids " + what)
             assert_same(bd.view(np.int32), do.view(np.int32), "batched dists " + what)
+
+
+@pytest.mark.parametrize("h2", [32, 256, 512])
+def test_ragged_batches_predictor(ctx, int_workload, h2):
+    """The entry predictor on 1..300-row batches (one partial row block of
+    the output-layer kernel: the eight-wave kernel at h2 = 256, the
+    four-wave one at 32 and 512; column groups of a single 64-column tile)
+    gives every row the leaf it gets inside a 600-row batch (the softmax
+    partials then combine in another grouping, so confidences agree to
+    rounding), and rte_predict_entry equals rte_select_entry followed by
+    rte_mlp_predict bit for bit."""
+    import torch
+    w = int_workload
+    _setup(ctx, w)
+    rng = np.random.default_rng(h2)
+    d = w.queries.shape[1]
+    ctx.mlp_create(d, 64, h2, 1000, seed=h2)
+    W, b = ctx.mlp_get()
+    ctx.mlp_set([x * 2 for x in W], [rng.normal(0, 0.3, size=x.shape).astype(np.float32)
+                                     for x in b])
+    X = torch.from_numpy((rng.normal(size=(600, d)) * 3).astype(np.float32)).cuda()
+    conf_all = torch.empty(600, dtype=torch.float32, device="cuda")
+    leaf_all = ctx.mlp_predict(X, min_conf=0.0, conf=conf_all).cpu().numpy()
+    conf_all = conf_all.cpu().numpy()
+    for nq in (1, 2, 63, 65, 127, 257, 300):
+        print(f"[ragged predictor] h2 {h2} nq {nq}", flush=True)
+        Xs = X[:nq].contiguous()
+        conf = torch.empty(nq, dtype=torch.float32, device="cuda")
+        leaf = ctx.mlp_predict(Xs, min_conf=0.0, conf=conf).cpu().numpy()
+        assert_same(leaf, leaf_all[:nq], f"leaves nq {nq}")
+        c = conf.cpu().numpy()
+        assert np.abs(c - conf_all[:nq]).max() <= 1e-5 * conf_all[:nq].max(), nq
+        kd = ctx.select_entry(Xs)
+        c0 = torch.empty(nq, dtype=torch.float32, device="cuda")
+        c1 = torch.empty(nq, dtype=torch.float32, device="cuda")
+        ref = ctx.mlp_predict(Xs, fallback=kd, min_conf=0.5, conf=c0)
+        got = ctx.predict_entry(Xs, min_conf=0.5, conf=c1)
+        ctx.synchronize()
+        assert_same(got.cpu().numpy(), ref.cpu().numpy(), f"predict_entry nq {nq}")
+        assert_same(c1.cpu().numpy().view(np.int32), c0.cpu().numpy().view(np.int32),
+                    f"predict_entry confidences nq {nq}")
