@@ -903,11 +903,13 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         assert np.array_equal(ids_h.numpy(), ids.cpu().numpy())
-        return {"qps": round(nq * args.steps / wall, 1),
-                "ms_per_step": round(wall / args.steps * 1e3, 4),
-                "h2d_bytes_per_step": int(Qh.numel() * 4),
-                "d2h_bytes_per_step": int(ids_h.numel() * 4 + dists_h.numel() * 4),
-                "memory": "pinned host buffers, copies on the query stream"}
+        out = {"qps": round(nq * args.steps / wall, 1),
+               "ms_per_step": round(wall / args.steps * 1e3, 4),
+               "h2d_bytes_per_step": int(Qh.numel() * 4),
+               "d2h_bytes_per_step": int(ids_h.numel() * 4 + dists_h.numel() * 4),
+               "memory": "pinned host buffers, copies on the query stream"}
+
+        return out
 
     host_io = measure_host_io(head) if not (sd or mixed) else None
     if host_io is not None:
