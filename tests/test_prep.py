@@ -191,3 +191,6 @@ def test_entry_links_reference_rules():
             assert not set(links) & set(a[u, :keep].tolist())
             assert all(abs(int(belong[v]) - int(belong[u])) <= w for v in links)
         assert (a[:, keep:] != n).mean() > 0.99
+        rows = np.array([0, 7, 7, 1234, n - 1])
+        assert_same(D.add_entry_links(g, pids, r, window=w, seed=9, rows=rows), a[rows],
+                    "a row subset equals those rows of the whole graph")
