@@ -642,6 +642,38 @@ __global__ void key_hist_kernel(KeySort k) {
 // [t*per, (t+1)*per) of the pass, a workgroup scan combines the sums, and
 // the prefixes leave with coalesced writes that also zero the histogram.
 constexpr int kScanChunk = 8192;
+
+// The counts vals[0, nc) of one pass (in LDS) become exclusive prefixes
+// plus carry, in place; returns carry plus the pass's total.  All 1024
+// threads of the workgroup call it (it holds barriers); thread t sums the
+// consecutive keys [t*per, (t+1)*per), a Hillis-Steele scan combines the
+// sums.
+__device__ __forceinline__ int32_t scan_pass(int32_t* vals, int32_t (*part)[1024], int nc,
+                                             int32_t carry, int t) {
+  const int per = (nc + 1023) / 1024;
+  const int k0 = min(nc, t * per), k1 = min(nc, k0 + per);
+  int32_t sum = 0;
+  for (int i = k0; i < k1; ++i) sum += vals[i];
+  part[0][t] = sum;
+  __syncthreads();
+  int src = 0;
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+    const int32_t v = part[src][t] + (t >= off ? part[src][t - off] : 0);
+    part[src ^ 1][t] = v;
+    __syncthreads();
+    src ^= 1;
+  }
+  int32_t run = carry + part[src][t] - sum;  // exclusive prefix of this thread's keys
+  for (int i = k0; i < k1; ++i) {
+    const int32_t h = vals[i];
+    vals[i] = run;
+    run += h;
+  }
+  const int32_t total = part[src][1023];
+  __syncthreads();
+  return carry + total;
+}
+
 __global__ __launch_bounds__(1024) void key_scan_kernel(KeySort k) {
   __shared__ int32_t vals[kScanChunk];
   __shared__ int32_t part[2][1024];
@@ -651,33 +683,31 @@ __global__ __launch_bounds__(1024) void key_scan_kernel(KeySort k) {
     const int nc = min(kScanChunk, k.n_keys - c0);
     for (int i = t; i < nc; i += 1024) vals[i] = k.hist[c0 + i];
     __syncthreads();
-    const int per = (nc + 1023) / 1024;
-    const int k0 = min(nc, t * per), k1 = min(nc, k0 + per);
-    int32_t sum = 0;
-    for (int i = k0; i < k1; ++i) sum += vals[i];
-    part[0][t] = sum;
-    __syncthreads();
-    int src = 0;
-    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
-      const int32_t v = part[src][t] + (t >= off ? part[src][t - off] : 0);
-      part[src ^ 1][t] = v;
-      __syncthreads();
-      src ^= 1;
-    }
-    int32_t run = carry + part[src][t] - sum;  // exclusive prefix of this thread's keys
-    for (int i = k0; i < k1; ++i) {
-      const int32_t h = vals[i];
-      vals[i] = run;
-      run += h;
-    }
-    carry += part[src][1023];
-    __syncthreads();
+    carry = scan_pass(vals, part, nc, carry, t);
     for (int i = t; i < nc; i += 1024) {
       k.cursor[c0 + i] = vals[i];
       k.hist[c0 + i] = 0;
     }
     __syncthreads();  // vals and part are rewritten by the next pass
   }
+  if (t < 16 * k.segments) k.seg[t] = 0;
+}
+
+// Histogram and scan in one workgroup, for batches small enough that one
+// workgroup reads every key quickly (nq <= kHistScanMaxQueries) and whose
+// keys fit one pass (n_keys <= kScanChunk): the counts are LDS atomics, the
+// global histogram is not touched (it stays zero), and a launch is saved.
+constexpr int64_t kHistScanMaxQueries = 32768;
+__global__ __launch_bounds__(1024) void key_hist_scan_kernel(KeySort k) {
+  __shared__ int32_t vals[kScanChunk];
+  __shared__ int32_t part[2][1024];
+  const int t = threadIdx.x;
+  for (int i = t; i < k.n_keys; i += 1024) vals[i] = 0;
+  __syncthreads();
+  for (int64_t q = t; q < k.nq; q += 1024) atomicAdd(&vals[sort_key(k, q)], 1);
+  __syncthreads();
+  scan_pass(vals, part, k.n_keys, 0, t);
+  for (int i = t; i < k.n_keys; i += 1024) k.cursor[i] = vals[i];
   if (t < 16 * k.segments) k.seg[t] = 0;
 }
 
@@ -743,8 +773,12 @@ static hipError_t enqueue_batched(rte_ctx* c, const rte::SearchArgs& a, const Ba
     if (bp.k_in)
       hipLaunchKernelGGL(batch_prep_kernel, dim3(nb), dim3(256), 0, c->stream, bp,
                          (int32_t*)a.k_per_q, (int32_t*)a.beam_per_q, (int64_t*)nullptr);
-    hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
-    hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(1024), 0, c->stream, *ks);
+    if (ks->n_keys <= kScanChunk && ks->nq <= kHistScanMaxQueries) {
+      hipLaunchKernelGGL(key_hist_scan_kernel, dim3(1), dim3(1024), 0, c->stream, *ks);
+    } else {
+      hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
+      hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(1024), 0, c->stream, *ks);
+    }
     hipLaunchKernelGGL(key_scatter_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
   }
   const hipError_t e1 = hipGetLastError();
