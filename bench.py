@@ -968,8 +968,9 @@ def main():
         g_h = pad_graph(g_h, W["n"])
         po = O.make_params(k=head["beam"] if sd else k, beam=head["beam"],
                            max_iter=head["max_iter"], mode=args.mode)
-        # every host thread this process is given: the GPU box grants one
-        # GPU's share of its cores through OMP_NUM_THREADS; nproc is recorded
+        # every host thread this process is given: a multi-GPU host grants
+        # each GPU's process its share of the cores through OMP_NUM_THREADS;
+        # nproc is recorded
         nproc = os.cpu_count() or 1
         threads = int(os.environ.get("OMP_NUM_THREADS") or nproc)
         cpu_model = ""
