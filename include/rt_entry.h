@@ -302,7 +302,9 @@ int rte_autotune_batched(rte_ctx* ctx, const float* PQ, int64_t nq, int64_t q_st
                          int n_bloom, rte_params* best, float* best_ms);
 
 /* Whole query path on device buffers (graph.cu:254-293 Graph::Search):
- * project, select entries (or take the predictor's leaves), search. */
+ * rte_project into PQ_scratch (nq x d_out), rte_select_entry into
+ * leaf_scratch when p->algo == 1 (the random entries of the other algo
+ * values use no leaves), then rte_search; same results as those calls. */
 int rte_query(rte_ctx* ctx, const float* Q, int64_t nq, const rte_params* p,
               float* PQ_scratch, int32_t* leaf_scratch, int32_t* ids_out,
               float* dists_out);

@@ -215,6 +215,27 @@ class Context:
             res.append(stats)
         return tuple(res) if len(res) > 1 else ids
 
+    def query(self, Q, params: Params):
+        """rte_query: the whole query path on device tensors in one call --
+        projection, kd-tree entry selection (ALGO 1; the random entries of
+        the other ALGO values need no leaves) and the search.  Returns
+        (ids, dists, PQ, leaf)."""
+        import torch
+        if not (_is_torch(Q) and Q.is_cuda):
+            raise ValueError("rte_query takes device tensors")
+        nq = Q.shape[0]
+        dev = Q.device
+        k = params.k if params.k > 0 else 1 << (max(params.beam, 1) - 1).bit_length()
+        pq = torch.empty((nq, self.d_out), dtype=torch.float32, device=dev)
+        leaf = torch.zeros(nq, dtype=torch.int32, device=dev)
+        ids = torch.empty((nq, k), dtype=torch.int32, device=dev)
+        dists = torch.empty((nq, k), dtype=torch.float32, device=dev)
+        qp, _, _ = _ptr(Q, np.float32)
+        P = lambda t: C.c_void_p(t.data_ptr())
+        self._chk(lib().rte_query(self._h, qp, nq, C.byref(params), P(pq), P(leaf), P(ids),
+                                  P(dists)))
+        return ids, dists, pq, leaf
+
     def search_batched(self, PQ, leaf, k_per_q, beam_per_q, params: Params, k_stride=None,
                        out=None, beam_bound=None):
         """Batched launch (an uneven batch: a captured hipGraph of the batch

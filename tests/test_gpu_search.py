@@ -406,3 +406,34 @@ def test_search_parity_small_beams(ctx, int_workload, beam, mode):
         assert_same(ig, io, f"ids {kw}")
         assert_same(dg.view(np.int32), do.view(np.int32), f"dists {kw}")
         assert_same(sg, so.astype(np.int32), f"stats {kw}")
+
+
+def test_query_one_call_equals_the_three_calls(ctx, int_workload):
+    """rte_query (projection, entry selection, search in one call) gives the
+    projection, leaves, ids and distances of rte_project + rte_select_entry +
+    rte_search bit for bit, and the oracle's ids on its PCA queries; with
+    random entries (ALGO 0) no leaves are selected."""
+    import torch
+    from rt_entry_amd import make_params
+    w = int_workload
+    _ctx_for(ctx, w)
+    ctx.set_pca(w.R, w.mu, w.R.shape[1])
+    Q = torch.from_numpy(w.raw_queries).cuda()
+    p = make_params(k=10, beam=64)
+    ids, dists, pq, leaf = ctx.query(Q, p)
+    pq2 = ctx.project(Q)
+    leaf2 = ctx.select_entry(pq2)
+    ids2, dists2 = ctx.search(pq2, leaf2, p)
+    ctx.synchronize()
+    assert_same(pq.cpu().numpy().view(np.int32), pq2.cpu().numpy().view(np.int32), "PCA queries")
+    assert_same(leaf.cpu().numpy(), leaf2.cpu().numpy(), "leaves")
+    assert_same(ids.cpu().numpy(), ids2.cpu().numpy(), "ids")
+    assert_same(dists.cpu().numpy().view(np.int32), dists2.cpu().numpy().view(np.int32), "dists")
+    io, do = O.search(w.base, w.graph, pq.cpu().numpy(), w.pids, leaf.cpu().numpy(),
+                      O.make_params(k=10, beam=64))
+    assert_same(ids.cpu().numpy(), io, "ids vs oracle")
+    p0 = make_params(k=10, beam=64, algo=0)
+    ids0, dists0, _, _ = ctx.query(Q, p0)
+    ids3, dists3 = ctx.search(pq2, None, p0)
+    ctx.synchronize()
+    assert_same(ids0.cpu().numpy(), ids3.cpu().numpy(), "random-entry ids")
