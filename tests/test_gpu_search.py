@@ -437,3 +437,19 @@ def test_query_one_call_equals_the_three_calls(ctx, int_workload):
     ids3, dists3 = ctx.search(pq2, None, p0)
     ctx.synchronize()
     assert_same(ids0.cpu().numpy(), ids3.cpu().numpy(), "random-entry ids")
+
+
+def test_caller_stream_is_used_and_restored():
+    """rte_set_stream makes the context use a caller stream (rte_get_stream
+    returns it) and NULL restores the context's own stream."""
+    import torch
+    from rt_entry_amd import Context
+    from rt_entry_amd._lib import lib
+    with Context(0) as cx:
+        own = lib().rte_get_stream(cx._h)
+        assert own
+        s = torch.cuda.Stream()
+        cx.set_stream(s.cuda_stream)
+        assert lib().rte_get_stream(cx._h) == s.cuda_stream
+        cx.set_stream(None)
+        assert lib().rte_get_stream(cx._h) == own
