@@ -453,3 +453,29 @@ def test_caller_stream_is_used_and_restored():
         assert lib().rte_get_stream(cx._h) == s.cuda_stream
         cx.set_stream(None)
         assert lib().rte_get_stream(cx._h) == own
+
+
+@pytest.mark.parametrize("d,nq,d_out", [(100, 1, 3), (100, 65, 100), (260, 130, 64),
+                                        (960, 67, 128), (64, 200, 37)])
+def test_projection_tolerance_odd_shapes(d, nq, d_out):
+    """The projection kernel on shapes off its fast paths: rows that are not
+    16-byte aligned (d % 4 != 0: scalar loads), several 128-deep chunks with
+    a partial last one (each split into two k halves), a partial last row
+    tile, output widths that are not multiples of 4 or of the 64-column tile
+    -- all within the FP32 GEMM error bound of the FP64 result."""
+    rng = np.random.default_rng(d + nq + d_out)
+    R = rng.normal(size=(d, d)).astype(np.float32) / np.sqrt(d)
+    mu = rng.normal(size=d).astype(np.float32)
+    Q = (rng.normal(size=(nq, d)) * 3).astype(np.float32)
+    from rt_entry_amd import Context
+    with Context(0) as cx:  # its own context: the shared one keeps its PCA
+        cx.set_pca(R, mu, d_out)
+        pq = cx.project(Q)
+    ref = O.project_f64(Q, R, mu, d_out)
+    bound = 4 * d * 2.0 ** -24 * (
+        np.abs(Q).astype(np.float64) @ np.abs(R[:, :d_out]).astype(np.float64)
+        + np.abs(mu).astype(np.float64) @ np.abs(R[:, :d_out]).astype(np.float64))
+    err = np.abs(pq.astype(np.float64) - ref)
+    print(f"[project] d {d} nq {nq} d_out {d_out} max err {err.max():.3e}", flush=True)
+    assert pq.shape == (nq, d_out)
+    assert (err <= bound + 1e-30).all(), f"max err {err.max()}"
