@@ -303,20 +303,34 @@ __device__ __forceinline__ void hidden_rows(const HiddenArgs& a, int64_t blk, ui
     bias1[c] = b1[min(col, d1 - 1)];
     bias2[c] = b2[min(col, d2 - 1)];
   }
-  {  // normalised bf16 input tile: 8 threads per row, float4 each
+  {  // normalised bf16 input tile: 8 threads per row, float4 each; the
+     // loads of four column steps (row, mean, 1/std) are all issued at
+     // clamped, valid addresses before any is used, so four steps cost one
+     // memory round trip instead of four
     const int row = t >> 3;
     const bool in = m0 + row < n;
-    const float* xr = X + (m0 + row) * ldx;
-    for (int c4 = (t & 7) * 4; c4 < d0; c4 += 32) {
-      uint2 o = make_uint2(0, 0);
-      if (in) {
-        const float4 x = *reinterpret_cast<const float4*>(xr + c4);
-        const float4 u = *reinterpret_cast<const float4*>(mean + c4);
-        const float4 v = *reinterpret_cast<const float4*>(inv + c4);
-        o.x = pack2((x.x - u.x) * v.x, (x.y - u.y) * v.y);
-        o.y = pack2((x.z - u.z) * v.z, (x.w - u.w) * v.w);
+    const float* xr = X + (in ? m0 + row : 0) * ldx;
+    for (int cb = (t & 7) * 4; cb < d0; cb += 4 * 32) {
+      float4 x[4], u[4], v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int c4 = min(cb + 32 * i, d0 - 4);
+        x[i] = *reinterpret_cast<const float4*>(xr + c4);
+        u[i] = *reinterpret_cast<const float4*>(mean + c4);
+        v[i] = *reinterpret_cast<const float4*>(inv + c4);
       }
-      *reinterpret_cast<uint2*>(&Xs[row * ld0 + c4]) = o;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int c4 = cb + 32 * i;
+        if (c4 < d0) {
+          uint2 o = make_uint2(0, 0);
+          if (in) {
+            o.x = pack2((x[i].x - u[i].x) * v[i].x, (x[i].y - u[i].y) * v[i].y);
+            o.y = pack2((x[i].z - u[i].z) * v[i].z, (x[i].w - u[i].w) * v[i].w);
+          }
+          *reinterpret_cast<uint2*>(&Xs[row * ld0 + c4]) = o;
+        }
+      }
     }
   }
   __syncthreads();
