@@ -290,7 +290,10 @@ int rte_search_batched_noalloc(rte_ctx* ctx, const float* PQ, int64_t nq, int64_
  * with leaf entries) -- and every filter size in bloom_words[0..n_bloom)
  * (0 = the mode's default), and writes the fastest configuration to *best
  * (a copy of *p with wpq, wg_waves, queue, grid, order and bloom_words set)
- * and its mean time to *best_ms.  A shape that fails is
+ * and its time to *best_ms.  Each shape is timed over three launches (the
+ * mean); the shapes within 5% of the fastest mean (at most six) are then
+ * timed again in nine interleaved rounds, one launch of each per round, and
+ * the lowest median wins (*best_ms is that median).  A shape that fails is
  * skipped; the call fails only when every shape did (with the first
  * failure).  Launch shape and order never change results; the filter size
  * can, so callers pass only sizes they accept.  ids_out/dists_out receive

@@ -517,9 +517,11 @@ class Context:
 
     def autotune_batched(self, PQ, leaf, k_per_q, beam_per_q, params: Params, k_stride=None,
                          bloom_words=(0,)):
-        """Fastest (wg_waves, grid, bloom_words) for this batch (device
-        tensors; k_per_q = beam_per_q = None for a uniform batch).  Returns
-        (best Params, best ms)."""
+        """Fastest launch shape (wpq, wg_waves, queue, grid, order,
+        bloom_words) for this batch (device tensors; k_per_q = beam_per_q =
+        None for a uniform batch): every shape timed over three launches,
+        the close finalists again in interleaved rounds.  Returns (best
+        Params, its time in ms)."""
         import torch
         nq, qs = PQ.shape
         uniform = k_per_q is None and beam_per_q is None
