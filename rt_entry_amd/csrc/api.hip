@@ -170,6 +170,8 @@ int set_device(rte_ctx* c) {
 
 }  // namespace
 
+static hipError_t set_sort_kernel_attributes();
+
 extern "C" {
 
 int rte_create(rte_ctx** out, int device) {
@@ -184,6 +186,11 @@ int rte_create(rte_ctx** out, int device) {
   // queues there (torch's default stream) and the context's kernels see each
   // other's results without explicit events
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->own) != hipSuccess) {
+    delete c;
+    return RTE_EHIP;
+  }
+  if (set_sort_kernel_attributes() != hipSuccess) {
+    (void)hipStreamDestroy(c->own);
     delete c;
     return RTE_EHIP;
   }
@@ -622,12 +629,17 @@ struct KeySort {
   int segments;
 };
 
-__device__ __forceinline__ int sort_key(const KeySort& k, int64_t q) {
-  int l = k.leaf ? k.leaf[q] : 0;
+// key of a query with entry leaf l and effective beam b (b unused without
+// beam classes)
+__device__ __forceinline__ int key_from(const KeySort& k, int l, int b) {
   if (l < 0 || l >= k.n_leaf) l = 0;  // the search's own clamp of a bad leaf
   if (!k.beam) return l;
-  const int cl = (31 - __clz(k.beam[q])) - 5;  // beams 32..512 -> 0..4
+  const int cl = (31 - __clz(b)) - 5;  // beams 32..512 -> 0..4
   return (4 - cl) * k.n_leaf + l;
+}
+
+__device__ __forceinline__ int sort_key(const KeySort& k, int64_t q) {
+  return key_from(k, k.leaf ? k.leaf[q] : 0, k.beam ? k.beam[q] : 32);
 }
 
 __global__ void key_hist_kernel(KeySort k) {
@@ -695,16 +707,40 @@ __global__ __launch_bounds__(1024) void key_scan_kernel(KeySort k) {
 
 // Histogram and scan in one workgroup, for batches small enough that one
 // workgroup reads every key quickly (nq <= kHistScanMaxQueries) and whose
-// keys fit one pass (n_keys <= kScanChunk): the counts are LDS atomics, the
-// global histogram is not touched (it stays zero), and a launch is saved.
+// counts fit its LDS (n_keys <= kHistScanMaxKeys, dynamic LDS of n_keys
+// words): the counts are LDS atomics, the global histogram is not touched
+// (it stays zero), the scan is one pass, and a launch is saved.
 constexpr int64_t kHistScanMaxQueries = 32768;
+constexpr int kHistScanMaxKeys = 32768;  // 128 KB of counts
 __global__ __launch_bounds__(1024) void key_hist_scan_kernel(KeySort k) {
-  __shared__ int32_t vals[kScanChunk];
+  extern __shared__ int32_t vals[];  // n_keys counts
   __shared__ int32_t part[2][1024];
   const int t = threadIdx.x;
   for (int i = t; i < k.n_keys; i += 1024) vals[i] = 0;
   __syncthreads();
-  for (int64_t q = t; q < k.nq; q += 1024) atomicAdd(&vals[sort_key(k, q)], 1);
+  // the leaves and beams of kU queries per thread are loaded (at clamped,
+  // valid indices) before any is counted: one memory round trip per kU
+  // queries instead of one per query
+  constexpr int kU = 8;
+  for (int64_t q0 = t; q0 < k.nq; q0 += 1024 * kU) {
+    int lf[kU], bm[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      lf[u] = 0;
+      bm[u] = 32;
+    }
+    if (k.leaf) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u) lf[u] = k.leaf[min(q0 + 1024 * u, k.nq - 1)];
+    }
+    if (k.beam) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u) bm[u] = k.beam[min(q0 + 1024 * u, k.nq - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (q0 + 1024 * u < k.nq) atomicAdd(&vals[key_from(k, lf[u], bm[u])], 1);
+  }
   __syncthreads();
   scan_pass(vals, part, k.n_keys, 0, t);
   for (int i = t; i < k.n_keys; i += 1024) k.cursor[i] = vals[i];
@@ -714,6 +750,14 @@ __global__ __launch_bounds__(1024) void key_hist_scan_kernel(KeySort k) {
 __global__ void key_scatter_kernel(KeySort k) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < k.nq) k.order[atomicAdd(&k.cursor[sort_key(k, q)], 1)] = (int32_t)q;
+}
+
+// Launch attributes of the kernels above, set on the current device when a
+// context is created (before any launch, so never inside a stream capture).
+static hipError_t set_sort_kernel_attributes() {
+  return hipFuncSetAttribute((const void*)key_hist_scan_kernel,
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(kHistScanMaxKeys * sizeof(int32_t)));
 }
 
 // cursors: classes in descending beam order
@@ -773,8 +817,9 @@ static hipError_t enqueue_batched(rte_ctx* c, const rte::SearchArgs& a, const Ba
     if (bp.k_in)
       hipLaunchKernelGGL(batch_prep_kernel, dim3(nb), dim3(256), 0, c->stream, bp,
                          (int32_t*)a.k_per_q, (int32_t*)a.beam_per_q, (int64_t*)nullptr);
-    if (ks->n_keys <= kScanChunk && ks->nq <= kHistScanMaxQueries) {
-      hipLaunchKernelGGL(key_hist_scan_kernel, dim3(1), dim3(1024), 0, c->stream, *ks);
+    if (ks->n_keys <= kHistScanMaxKeys && ks->nq <= kHistScanMaxQueries) {
+      hipLaunchKernelGGL(key_hist_scan_kernel, dim3(1), dim3(1024),
+                         (size_t)ks->n_keys * sizeof(int32_t), c->stream, *ks);
     } else {
       hipLaunchKernelGGL(key_hist_kernel, dim3(nb), dim3(256), 0, c->stream, *ks);
       hipLaunchKernelGGL(key_scan_kernel, dim3(1), dim3(1024), 0, c->stream, *ks);

@@ -239,3 +239,43 @@ def test_autotune_explores_new_knobs(ctx, int_workload):
         ctx.search(PQ, leaf, make_params(k=10, beam=64, queue=2))
     with pytest.raises(RteError):  # the LDS queue runs one wave per query
         ctx.search(PQ, leaf, make_params(k=10, beam=64, queue=1, wpq=2))
+
+
+def test_batched_uneven_leaf_order_many_keys(ctx, int_workload):
+    """Uneven batch sorted by (beam class, entry leaf) over a kd-tree of
+    two-point leaves: about 4000 leaves, so about 20,000 sort keys, whose
+    histogram and scan run in one workgroup with the counts in dynamic LDS
+    (above 8192 keys, which fit the static array of the multi-pass scan).
+    The oracle's per-query results."""
+    import torch
+    from rt_entry_amd import data as D
+    from rt_entry_amd import make_params
+    w = int_workload
+    boxes, belong = O.kdtree_build(w.base[:, :3], 2)
+    n_leaf = boxes.shape[0]
+    assert 8192 < 5 * n_leaf <= 32768, n_leaf
+    pids = O.aabb_pids(belong, n_leaf, 2)
+    aabbs = O.expand_aabbs(boxes, 0.2)
+    hits = O.select_entry(aabbs, w.queries)
+    ctx.set_base(w.base)
+    ctx.set_graph(w.graph_raw)
+    ctx.set_leaves(aabbs, pids)
+    try:
+        dev = torch.device("cuda:0")
+        nq = w.queries.shape[0]
+        kq, bq = D.k_beam_assignment(4, nq)
+        io, do = O.search_uneven(w.base, w.graph, w.queries, pids, hits, kq, bq,
+                                 O.make_params(k=10, beam=64, mode=0))
+        print(f"[variant] uneven leaf order, {n_leaf} leaves, {5 * n_leaf} keys", flush=True)
+        ig, dg = ctx.search_batched(torch.from_numpy(w.queries).to(dev),
+                                    torch.from_numpy(hits).to(dev),
+                                    torch.from_numpy(kq).to(dev), torch.from_numpy(bq).to(dev),
+                                    make_params(mode=0, order=1), k_stride=io.shape[1])
+        ctx.synchronize()
+        ig, dg = ig.cpu().numpy(), dg.cpu().numpy()
+        for q in range(nq):
+            k = kq[q]
+            assert_same(ig[q, :k], io[q, :k], f"ids q={q} k={k} beam={bq[q]}")
+            assert_same(dg[q, :k].view(np.int32), do[q, :k].view(np.int32), f"dists q={q}")
+    finally:
+        ctx.set_leaves(w.aabbs, w.pids)
