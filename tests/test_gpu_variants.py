@@ -246,7 +246,9 @@ def test_batched_uneven_leaf_order_many_keys(ctx, int_workload):
     two-point leaves: about 4000 leaves, so about 20,000 sort keys, whose
     histogram and scan run in one workgroup with the counts in dynamic LDS
     (above 8192 keys, which fit the static array of the multi-pass scan).
-    The oracle's per-query results."""
+    The 256 queries give the oracle's per-query results; the same queries
+    repeated 80 times (20,480 queries, close to the batch size that path
+    takes) give the same results per query."""
     import torch
     from rt_entry_amd import data as D
     from rt_entry_amd import make_params
@@ -277,5 +279,21 @@ def test_batched_uneven_leaf_order_many_keys(ctx, int_workload):
             k = kq[q]
             assert_same(ig[q, :k], io[q, :k], f"ids q={q} k={k} beam={bq[q]}")
             assert_same(dg[q, :k].view(np.int32), do[q, :k].view(np.int32), f"dists q={q}")
+        rep = 80
+        assert rep * nq <= 32768
+        print(f"[variant] the same batch repeated {rep} times: {rep * nq} queries", flush=True)
+        it, dt = ctx.search_batched(torch.from_numpy(np.tile(w.queries, (rep, 1))).to(dev),
+                                    torch.from_numpy(np.tile(hits, rep)).to(dev),
+                                    torch.from_numpy(np.tile(kq, rep)).to(dev),
+                                    torch.from_numpy(np.tile(bq, rep)).to(dev),
+                                    make_params(mode=0, order=1), k_stride=io.shape[1])
+        ctx.synchronize()
+        it, dt = it.cpu().numpy(), dt.cpu().numpy()
+        mask = np.arange(io.shape[1])[None, :] < kq[:, None]
+        for r in range(rep):
+            blk = slice(r * nq, (r + 1) * nq)
+            assert_same(np.where(mask, it[blk], 0), np.where(mask, ig, 0), f"ids, copy {r}")
+            assert_same(np.where(mask, dt[blk].view(np.int32), 0),
+                        np.where(mask, dg.view(np.int32), 0), f"dists, copy {r}")
     finally:
         ctx.set_leaves(w.aabbs, w.pids)
